@@ -1,0 +1,999 @@
+/* CPU oracle, part 2: type-1 / type-2 integrals, the binomial shift and the
+ * triplet driver (TEST INFRASTRUCTURE).  Same arithmetic order as the
+ * reference (src/type1.c, src/type2.c, src/util.c, src/angular_integrals.c,
+ * src/libecp.c, src/getIntegrals.c).  Error paths follow libecp.h:23-26.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle_internal.h"
+
+struct orc_handle {
+  int nrAtoms;
+  const double *geom;
+  orc_ecp **U;                 /* per atom, NULL if no ECP */
+  const int *shells, *am, *K;  /* basis, borrowed */
+  const double *d, *a;
+  int nrShells, maxShells;
+  int deriv;
+  int maxLECP, maxLBS, maxAlpha, maxLambda;
+  int *maxLAtom;
+  double *fac, *dfac;
+  int *ijk, *ijkIndex, ijkDim, ownIJK;
+  int tmDim;
+  double *cart2sph, *poly2sph;
+  orc_grid *largeGrid;
+  orc_bessel bessel;
+  int haveBessel;
+  orc_grid *grid1, *grid2;     /* small PS93 grids of type 1 and type 2 */
+  double tolerance, accuracy, lnAcc1;
+  double acc2, lnAcc2;         /* type-2 numerical zero, fixed at 1e-14 (type2.c:58-59) */
+  int options;
+  orc_stats st;
+};
+
+/* ---------------------------------------------------------------------- */
+/* small n-d arrays                                                        */
+
+typedef struct {
+  int d[4];
+  double *a;
+} arr_t;
+
+static arr_t arr_new(int n0, int n1, int n2, int n3) {
+  arr_t r;
+  r.d[0] = n0; r.d[1] = n1; r.d[2] = n2; r.d[3] = n3;
+  r.a = calloc((size_t)n0 * n1 * n2 * n3, sizeof(double));
+  return r;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Angular integrals                                                        */
+
+/* Omega[LM(l,m)][LM(lambda,mu)][C(alpha,c)] = int S_lm S_lambda,mu x^i y^j z^k
+   over the unit sphere for l < maxL, lambda <= maxLambda, alpha <= maxAlpha;
+   angular_integrals.c:15-101. */
+static arr_t tab_omega(const orc_handle *h) {
+  const int maxL = h->maxLECP;
+  arr_t O = arr_new(ODIM_L(maxL), ODIM_L(h->maxLambda), ODIM_C(h->maxAlpha), 1);
+  const int inc1 = O.d[2], inc2 = O.d[1] * O.d[2];
+  const int tmd = ODIM_L(h->tmDim);
+  const int *ijk = h->ijk;
+  for (int lambda = 0; lambda <= h->maxLambda; lambda++) {
+    for (int l = 0; l < maxL; l++) {
+      int i = (lambda + l) % 2, j = lambda - l;
+      int minAlpha = (i > j) ? i : j;
+      for (int alpha = minAlpha; alpha <= h->maxAlpha; alpha += 2) {
+        for (int mu = 0; mu < 2 * lambda + 1; mu++) {
+          for (int m = 0; m < 2 * l + 1; m++) {
+            double *cijk = O.a + OIDX_LM(l, m) * inc2 + OIDX_LM(lambda, mu) * inc1 + ODIM_C(alpha - 1);
+            for (int c = 0; c < ODIM_IJK(alpha); c++) {
+              int p = 3 * OIDX_C(alpha, c);
+              int ii = ijk[p], jj = ijk[p + 1], kk = ijk[p + 2];
+              if (0 == alpha) {
+                if (l == lambda && m == mu) cijk[c] = 1.0;
+              } else if (lambda <= l + alpha) {
+                for (int dd = 0; dd < ODIM_IJK(l); dd++) {
+                  int q = 3 * OIDX_C(l, dd);
+                  int ax = ijk[q], ay = ijk[q + 1], az = ijk[q + 2];
+                  double N = 0.25 * h->dfac[2 * l + 1] / M_PI;
+                  if (ax > 1) N /= h->dfac[2 * ax - 1];
+                  if (ay > 1) N /= h->dfac[2 * ay - 1];
+                  if (az > 1) N /= h->dfac[2 * az - 1];
+                  N = sqrt(N);
+                  ax += ii; ay += jj; az += kk;
+                  int pp = h->ijkIndex[ax * h->ijkDim * h->ijkDim + ay * h->ijkDim + az] * tmd +
+                           OIDX_LM(lambda, mu);
+                  int tmi = (l == 0) ? 0 : orc_tm_offset(l) + m * ODIM_IJK(l) + dd;
+                  cijk[c] += N * h->cart2sph[tmi] * h->poly2sph[pp];
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  return O;
+}
+
+/* ang[lambda][LM(l,m)][C(la,c)] = sum_mu S_lambda,mu(rA^) Omega[..];
+   angular_integrals.c:104-142. */
+static arr_t eval_angular(const orc_handle *h, const arr_t *O, int lmax_a, const double *rA) {
+  const int maxL = h->maxLECP;
+  const int lmax = maxL - 1 + lmax_a;
+  arr_t A = arr_new(lmax + 1, ODIM_L(h->maxLambda), ODIM_C(lmax_a), 1);
+  const int inc1 = O->d[2], inc2 = O->d[1] * O->d[2];
+  const int incA1 = A.d[2], incA2 = A.d[1] * A.d[2];
+  double R[3];
+  double *rsph = calloc((size_t)ODIM_L(lmax), sizeof(double));
+  orc_sph_coords(rA, R);
+  orc_rsh(lmax, R[1], R[2], h->fac, h->dfac, rsph);
+  for (int l = 0; l < maxL; l++)
+    for (int m = 0; m < 2 * l + 1; m++)
+      for (int lambda = 0; lambda <= lmax; lambda++)
+        for (int la = 0; la <= lmax_a; la++)
+          for (int c = 0; c < ODIM_IJK(la); c++) {
+            double v = 0.0;
+            for (int mu = 0; mu < 2 * lambda + 1; mu++)
+              v += rsph[OIDX_LM(lambda, mu)] *
+                   O->a[OIDX_LM(l, m) * inc2 + OIDX_LM(lambda, mu) * inc1 + OIDX_C(la, c)];
+            A.a[lambda * incA2 + OIDX_LM(l, m) * incA1 + OIDX_C(la, c)] = v;
+          }
+  free(rsph);
+  return A;
+}
+
+/* (C - A)^ijk table: the factors of the binomial shift; util.c:214-243. */
+static arr_t usp_table(int l, const double *r) {
+  const int dim = l + 1;
+  arr_t t = arr_new(dim, dim, dim, 1);
+  double x = 1.0, y = 1.0, z = 1.0;
+  for (int i = 0; i <= l; i++) {
+    x = (0 == i) ? 1.0 : -x * r[0];
+    for (int j = 0; j <= l - i; j++) {
+      y = (0 == j) ? 1.0 : -y * r[1];
+      for (int k = 0; k <= l - i - j; k++) {
+        z = (0 == k) ? 1.0 : -z * r[2];
+        t.a[i * dim * dim + j * dim + k] = x * y * z;
+      }
+    }
+  }
+  return t;
+}
+
+static double binom(int n, int k, const double *fac) {
+  return (k >= 0 && k <= n) ? fac[n] / (fac[n - k] * fac[k]) : 0.0;
+}
+
+/* Two-stage binomial shift of a block over C-centred monomials to the A-
+   and B-centred Cartesian functions, times the norm N; util.c:246-334.
+   Returns an IJK(la) x IJK(lb) block. */
+static double *shift_block(const orc_handle *h, const double *gamma, int gcols, double tol, double N,
+                           int la, const arr_t *uA, int lb, const arr_t *uB) {
+  const int *ijk = h->ijk, *idx = h->ijkIndex, D = h->ijkDim;
+  const int incA1 = uA->d[2], incA2 = uA->d[1] * uA->d[2];
+  const int incB1 = uB->d[2], incB2 = uB->d[1] * uB->d[2];
+  const int incJ = ODIM_C(lb), incI = ODIM_IJK(lb);
+  double *J = calloc((size_t)ODIM_IJK(la) * ODIM_C(lb), sizeof(double));
+  double *I = calloc((size_t)ODIM_IJK(la) * ODIM_IJK(lb), sizeof(double));
+  for (int c1 = 0; c1 < ODIM_IJK(la); c1++) {
+    int p = 3 * OIDX_C(la, c1);
+    int ax = ijk[p], ay = ijk[p + 1], az = ijk[p + 2];
+    for (int alx = 0; alx <= ax; alx++) {
+      double bx = binom(ax, alx, h->fac);
+      for (int aly = 0; aly <= ay; aly++) {
+        double by = bx * binom(ay, aly, h->fac);
+        for (int alz = 0; alz <= az; alz++) {
+          double bz = by * binom(az, alz, h->fac);
+          double factor = bz * uA->a[(ax - alx) * incA2 + (ay - aly) * incA1 + (az - alz)];
+          if (fabs(factor) <= tol) continue;
+          int r = idx[alx * D * D + aly * D + alz];
+          for (int beta = 0; beta <= lb; beta++)
+            for (int c2 = 0; c2 < ODIM_IJK(beta); c2++) {
+              int q = OIDX_C(beta, c2);
+              J[c1 * incJ + q] += factor * gamma[r * gcols + q];
+            }
+        }
+      }
+    }
+  }
+  for (int c2 = 0; c2 < ODIM_IJK(lb); c2++) {
+    int q = 3 * OIDX_C(lb, c2);
+    int bx = ijk[q], by = ijk[q + 1], bz = ijk[q + 2];
+    for (int btx = 0; btx <= bx; btx++) {
+      double fx = binom(bx, btx, h->fac);
+      for (int bty = 0; bty <= by; bty++) {
+        double fy = fx * binom(by, bty, h->fac);
+        for (int btz = 0; btz <= bz; btz++) {
+          double fz = fy * binom(bz, btz, h->fac);
+          double factor = fz * uB->a[(bx - btx) * incB2 + (by - bty) * incB1 + (bz - btz)];
+          if (fabs(factor) <= tol) continue;
+          factor *= N;
+          int r = idx[btx * D * D + bty * D + btz];
+          for (int c1 = 0; c1 < ODIM_IJK(la); c1++) I[c1 * incI + c2] += factor * J[c1 * incJ + r];
+        }
+      }
+    }
+  }
+  free(J);
+  return I;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Type 1 (local channel)                                                  */
+
+typedef struct {
+  double C;
+  const double *expo, *K, *U, *rn;
+  double minExp;
+} q_params;
+
+/* C r^l1 U_L(r) K_l2(s r) exp(expo); type1.c:78-88 */
+static double q_integrand(int index, const void *vp) {
+  const q_params *p = vp;
+  double Q = 0.0;
+  double e = p->expo[index];
+  if (e >= p->minExp) Q = p->C * p->rn[index] * p->U[index] * p->K[index] * exp(e);
+  return Q;
+}
+
+/* Q[l1][l2], l2 <= l1 <= lab with l1 - l2 even: PS93 on the KK grid window,
+   failed entries redone per primitive pair on the FM06-mapped PSM92 grid;
+   type1.c:94-208.  Returns 0, or 1 on a PSM92 failure. */
+static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, int lab, double dAC,
+                  double ca, double za, double dBC, double cb, double zb, double *T) {
+  orc_grid *g1 = h->grid1;
+  const int so = g1->order, lo = h->largeGrid->order;
+  const int Tinc = lab + 1;
+  double zd2 = -za * dAC * dAC - zb * dBC * dBC;
+  double z = -za - zb;
+  int nFailed = 0, rc = 0;
+  int *failed = calloc(2 * (size_t)(lab + 1) * (lab + 1), sizeof(int));
+  double *expo = calloc((size_t)lo, sizeof(double));
+  double *Ub = calloc((size_t)lo, sizeof(double));
+  double *K = calloc((size_t)(lab + 1) * lo, sizeof(double));
+  double *rn = calloc((size_t)(lab + 1) * lo, sizeof(double));
+  q_params ps;
+  ps.C = ca * cb * exp(zd2);
+  ps.expo = expo;
+  ps.minExp = h->lnAcc1;
+  ps.U = Ub;
+  for (int n = g1->start; n < g1->end; n++) {
+    double r = g1->x[n];
+    expo[n] = (z * r + s) * r;
+    Ub[n] = U_L[n];
+    orc_bessel_weighted(&h->bessel, so, lab, s * r, K + n);
+    rn[n] = 1.0;
+    for (int l1 = 1; l1 <= lab; l1++) rn[l1 * so + n] = r * rn[(l1 - 1) * so + n];
+  }
+  for (int l1 = 0; l1 <= lab; l1++) {
+    ps.rn = rn + l1 * so;
+    for (int l2 = l1; l2 >= 0; l2 -= 2) {
+      ps.K = K + l2 * so;
+      h->st.ps93_calls[0]++;
+      if (0 != orc_integrate_ps93(q_integrand, &ps, g1)) {
+        h->st.ps93_fail[0]++;
+        failed[2 * nFailed] = l1;
+        failed[2 * nFailed + 1] = l2;
+        nFailed++;
+      } else {
+        T[l1 * Tinc + l2] += g1->I;
+      }
+    }
+  }
+  if (nFailed > 0) {
+    orc_grid *g = orc_grid_copy(h->largeGrid);
+    double zeta_p = za + zb;
+    double P = (za * dAC + zb * dBC) / zeta_p;
+    if (h->options & ORC_OPT_CLEAN_FALLBACK) {
+      memset(expo, 0, sizeof(double) * (size_t)lo);
+      memset(Ub, 0, sizeof(double) * (size_t)lo);
+    }
+    ps.C = ca * cb;
+    orc_map_fm06(g, zeta_p, P);
+    for (int n = 0; n < g->order; n++) {
+      double r = g->x[n];
+      expo[n] = (z * r + s) * r + zd2;
+      if (r > dAC && r > dBC && expo[n] < ps.minExp) {
+        g->end = n - 1;
+        if (n - 1 < (g->order - 1) / 2) h->st.left_breaks[0]++;
+        if (h->options & ORC_OPT_CLEAN_FALLBACK)
+          for (int m = n + 1; m < g->order; m++) expo[m] = -INFINITY;
+        break;
+      } else if (expo[n] >= ps.minExp) {
+        orc_bessel_weighted(&h->bessel, g->order, lab, s * r, K + n);
+        Ub[n] = orc_eval_ecp(U, U->L, r);
+        rn[n] = 1.0;
+        for (int l1 = 1; l1 <= lab; l1++) rn[l1 * g->order + n] = r * rn[(l1 - 1) * g->order + n];
+      }
+    }
+    for (int i = 0; i < nFailed; i++) {
+      int l1 = failed[2 * i], l2 = failed[2 * i + 1];
+      ps.rn = rn + l1 * g->order;
+      ps.K = K + l2 * g->order;
+      h->st.psm92_calls[0]++;
+      if (0 != orc_integrate_psm92(q_integrand, &ps, g)) {
+        h->st.psm92_fail[0]++;
+        rc = 1;
+        break;
+      }
+      T[l1 * Tinc + l2] += g->I;
+    }
+    orc_grid_free(g);
+  }
+  free(failed);
+  free(expo);
+  free(Ub);
+  free(K);
+  free(rn);
+  return rc;
+}
+
+/* Type-1 block over C-centred monomials, C_DIM(la) x C_DIM(lb);
+   type1.c:211-302.  Returns NULL on failure. */
+static double *calc_chi(orc_handle *h, const double *U_L, const orc_ecp *U, const double *rAC, double dAC,
+                        const double *rBC, double dBC, int la, int sha, int offa, int shifta, int lb,
+                        int shb, int offb, int shiftb) {
+  const int Na = h->K[sha], Nb = h->K[shb];
+  const double *za_ = h->a + offa, *ca_ = h->d + offa;
+  const double *zb_ = h->a + offb, *cb_ = h->d + offb;
+  const int lab = la + lb;
+  const int D = h->ijkDim;
+  const int tmd = ODIM_L(h->tmDim);
+  const int inc = ODIM_C(lb);
+  double *chi = calloc((size_t)ODIM_C(la) * ODIM_C(lb), sizeof(double));
+  double *rsph = calloc((size_t)ODIM_L(lab), sizeof(double));
+  double *Q = calloc((size_t)(lab + 1) * (lab + 1), sizeof(double));
+  for (int pa = 0; pa < Na; pa++) {
+    double za = za_[pa], ca = ca_[pa];
+    for (int p = 0; p < shifta; p++) ca *= za;
+    for (int pb = 0; pb < Nb; pb++) {
+      double zb = zb_[pb], cb = cb_[pb];
+      double P[3], S[3];
+      for (int p = 0; p < shiftb; p++) cb *= zb;
+      for (int p = 0; p < 3; p++) P[p] = 2.0 * (za * rAC[p] + zb * rBC[p]);
+      orc_sph_coords(P, S);
+      orc_rsh(lab, S[1], S[2], h->fac, h->dfac, rsph);
+      memset(Q, 0, sizeof(double) * (size_t)(lab + 1) * (lab + 1));
+      if (calc_q(h, U_L, U, S[0], lab, dAC, ca, za, dBC, cb, zb, Q)) {
+        free(chi);
+        free(rsph);
+        free(Q);
+        return NULL;
+      }
+      for (int ax = 0; ax <= la; ax++)
+        for (int ay = 0; ay <= la - ax; ay++)
+          for (int az = 0; az <= la - ax - ay; az++) {
+            int i = h->ijkIndex[ax * D * D + ay * D + az];
+            for (int bx = 0; bx <= lb; bx++) {
+              int lx = ax + bx;
+              for (int by = 0; by <= lb - bx; by++) {
+                int ly = ay + by;
+                for (int bz = 0; bz <= lb - bx - by; bz++) {
+                  int j = h->ijkIndex[bx * D * D + by * D + bz];
+                  int lz = az + bz;
+                  int lmax = lx + ly + lz;
+                  int p = h->ijkIndex[lx * D * D + ly * D + lz];
+                  for (int l = lmax; l >= 0; l -= 2) {
+                    double factor = 0.0;
+                    for (int m = 0; m < 2 * l + 1; m++)
+                      factor += rsph[OIDX_LM(l, m)] * h->poly2sph[p * tmd + OIDX_LM(l, m)];
+                    chi[i * inc + j] += factor * Q[lmax * (lab + 1) + l];
+                  }
+                }
+              }
+            }
+          }
+    }
+  }
+  free(rsph);
+  free(Q);
+  return chi;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Type 2 (semi-local projectors)                                          */
+
+typedef struct {
+  int start, end, skip;
+} screen_t;
+
+/* Grid window of points within shellRadius of d_IC; type2.c:148-180. */
+static screen_t basis_screening(const orc_grid *g, double R, double d) {
+  const double rmin = d - R, rmax = d + R;
+  screen_t s;
+  int j;
+  for (j = g->end; j >= g->start; j--)
+    if (g->x[j] < rmin) break;
+  s.start = j + 1;
+  for (j = g->end; j >= g->start; j--)
+    if (g->x[j] <= rmax) break;
+  s.end = (j < 0 || g->x[j] > rmax) ? -1 : j;
+  s.skip = !(s.end >= s.start);
+  return s;
+}
+
+/* Last index, searching down from end, with |U| > zero, else -1;
+   util.c:198-210. */
+static int potential_screening(const orc_grid *g, const double *U, double zero) {
+  for (int i = g->end; i >= g->start; i--)
+    if (fabs(U[i]) > zero) return i;
+  return -1;
+}
+
+/* U_tab[l][k][n] = r_n^k U_l(r_n) for l < L, with the window end shrunk
+   channel by channel; type2.c:184-219. */
+static arr_t tab_ecp(const orc_ecp *U, orc_grid *g, int lmax, double tol) {
+  arr_t T = arr_new(U->L, lmax + 1, g->order, 1);
+  const int inc1 = T.d[2], inc2 = T.d[1] * T.d[2];
+  double *Ul = calloc((size_t)g->order, sizeof(double));
+  for (int l = 0; l < U->L; l++) {
+    for (int n = 0; n < g->order; n++) Ul[n] = orc_eval_ecp(U, l, g->x[n]);
+    g->end = potential_screening(g, Ul, tol);
+    for (int n = g->start; n <= g->end; n++) {
+      double *P = T.a + l * inc2 + n;
+      P[0] = Ul[n];
+      for (int k = 1; k <= lmax; k++) P[k * inc1] = P[(k - 1) * inc1] * g->x[n];
+    }
+  }
+  free(Ul);
+  return T;
+}
+
+/* F[run][shell][lambda][n] = sum_p d_p zeta_p^run K_lambda(2 zeta_p d_AC r)
+   exp(-zeta_p (d_AC - r)^2) on each shell's screened window;
+   type2.c:222-310. */
+static arr_t calc_f(orc_handle *h, const double *rC, screen_t **psg, int *err) {
+  const int nAt = h->nrAtoms, dim = h->maxShells;
+  const int nRuns = 1 + h->deriv;
+  const int lmax = h->maxLECP - 1;
+  orc_grid *g = h->grid2;
+  screen_t *sg = calloc((size_t)h->nrShells + 1, sizeof(screen_t));
+  int *prim = calloc((size_t)nAt * dim + 1, sizeof(int));
+  int *skipAtom = calloc((size_t)nAt, sizeof(int));
+  arr_t F = arr_new(nRuns, h->nrShells, h->maxLambda + 1, g->order);
+  const int inc1 = F.d[3], inc2 = F.d[2] * F.d[3], inc3 = F.d[1] * F.d[2] * F.d[3];
+  double *K = calloc((size_t)h->maxLambda + 1, sizeof(double));
+  int offset = 0, index = 0;
+  *psg = sg;
+  *err = 0;
+  for (int A = 0; A < nAt; A++) {
+    const double *rA = h->geom + 3 * A;
+    double dx = rC[0] - rA[0], dy = rC[1] - rA[1], dz = rC[2] - rA[2];
+    double dAC = sqrt(dx * dx + dy * dy + dz * dz);
+    skipAtom[A] = 1;
+    for (int sa = 0; sa < h->shells[A]; sa++) {
+      int ok;
+      double e = orc_shell_radius_(h->K[index], h->am[index], h->d + offset, h->a + offset, h->acc2, &ok);
+      if (!ok) *err = 1;
+      sg[index] = basis_screening(g, e, dAC);
+      if (!sg[index].skip) skipAtom[A] = 0;
+      prim[A * dim + sa] = offset;
+      offset += h->K[index];
+      index++;
+    }
+  }
+  for (int run = 0; run < nRuns; run++) {
+    index = -1;
+    for (int A = 0; A < nAt; A++) {
+      if (skipAtom[A]) {
+        index += h->shells[A];
+        continue;
+      }
+      const double *rA = h->geom + 3 * A;
+      double dx = rC[0] - rA[0], dy = rC[1] - rA[1], dz = rC[2] - rA[2];
+      double dAC = sqrt(dx * dx + dy * dy + dz * dz);
+      for (int sa = 0; sa < h->shells[A]; sa++) {
+        index++;
+        if (sg[index].skip) continue;
+        int lmaxA = lmax + h->am[index] + run;
+        for (int pa = 0; pa < h->K[index]; pa++) {
+          int off = prim[A * dim + sa] + pa;
+          double zeta = h->a[off], da = h->d[off];
+          for (int n = 0; n < run; n++) da *= zeta;
+          for (int n = sg[index].start; n < sg[index].end; n++) {
+            orc_bessel_weighted(&h->bessel, 1, lmaxA, 2.0 * zeta * dAC * g->x[n], K);
+            double e = dAC - g->x[n];
+            e = exp(-zeta * e * e);
+            for (int i = 0; i <= lmaxA; i++) F.a[run * inc3 + index * inc2 + i * inc1 + n] += da * K[i] * e;
+          }
+        }
+      }
+    }
+  }
+  free(skipAtom);
+  free(prim);
+  free(K);
+  return F;
+}
+
+typedef struct {
+  const double *Fa, *Fb, *U;
+} t_params;
+
+static double t_integrand(int n, const void *vp) {
+  const t_params *p = vp;
+  return p->Fa[n] * p->Fb[n] * p->U[n];
+}
+
+typedef struct {
+  double C;
+  const double *expo, *Ka, *Kb, *U, *rn;
+  double minExp;
+} q2_params;
+
+static double q2_integrand(int index, const void *vp) {
+  const q2_params *p = vp;
+  double Q = 0.0;
+  double e = p->expo[index];
+  if (e >= p->minExp) Q = p->C * p->U[index] * p->rn[index] * p->Ka[index] * p->Kb[index] * exp(e);
+  return Q;
+}
+
+/* Per-primitive-pair PSM92 fallback for the failed T entries, on FM06-mapped
+   grids; buffers persist across primitive pairs as in type2.c:417-528.
+   Returns 0, or 2 on a PSM92 failure. */
+static int calc_t_fallback(orc_handle *h, double *T, int Tinc1, int Tinc2, const int *fl, int nf,
+                           const orc_ecp *U, int l, double dAC, int alpha, int sha, int offa, int shifta,
+                           double dBC, int beta, int shb, int offb, int shiftb) {
+  const int Na = h->K[sha], Nb = h->K[shb];
+  const double *zs_a = h->a + offa, *cs_a = h->d + offa;
+  const double *zs_b = h->a + offb, *cs_b = h->d + offb;
+  const int laC = alpha + l, lbC = beta + l, lab = alpha + beta;
+  const int lo = h->largeGrid->order;
+  int rc = 0;
+  double *expo = calloc((size_t)lo, sizeof(double));
+  double *Ub = calloc((size_t)lo, sizeof(double));
+  double *Ka = calloc((size_t)(laC + 1) * lo, sizeof(double));
+  double *Kb = calloc((size_t)(lbC + 1) * lo, sizeof(double));
+  double *rn = calloc((size_t)(lab + 1) * lo, sizeof(double));
+  q2_params ps;
+  ps.expo = expo;
+  ps.minExp = h->lnAcc2;
+  ps.U = Ub;
+  for (int pa = 0; pa < Na && !rc; pa++) {
+    double zeta_a = zs_a[pa];
+    double s1 = 2.0 * zeta_a * dAC;
+    for (int pb = 0; pb < Nb && !rc; pb++) {
+      double zeta_b = zs_b[pb];
+      double s2 = 2.0 * zeta_b * dBC;
+      ps.C = cs_a[pa] * cs_b[pb];
+      for (int n = 0; n < shifta; n++) ps.C *= zeta_a;
+      for (int n = 0; n < shiftb; n++) ps.C *= zeta_b;
+      orc_grid *g = orc_grid_copy(h->largeGrid);
+      double zeta_p = zeta_a + zeta_b;
+      double P = (zeta_a * dAC + zeta_b * dBC) / zeta_p;
+      orc_map_fm06(g, zeta_p, P);
+      if (h->options & ORC_OPT_CLEAN_FALLBACK) memset(expo, 0, sizeof(double) * (size_t)lo);
+      for (int n = 0; n < g->order; n++) {
+        double r = g->x[n];
+        double d1 = dAC - r, d2 = dBC - r;
+        expo[n] = -zeta_a * d1 * d1 - zeta_b * d2 * d2;
+        if (r > dAC && r > dBC && expo[n] < ps.minExp) {
+          g->end = n - 1;
+          if (n - 1 < (g->order - 1) / 2) h->st.left_breaks[1]++;
+          if (h->options & ORC_OPT_CLEAN_FALLBACK)
+            for (int m = n + 1; m < g->order; m++) expo[m] = -INFINITY;
+          break;
+        } else if (expo[n] >= ps.minExp) {
+          Ub[n] = orc_eval_ecp(U, l, r);
+          orc_bessel_weighted(&h->bessel, g->order, laC, s1 * r, Ka + n);
+          orc_bessel_weighted(&h->bessel, g->order, lbC, s2 * r, Kb + n);
+          rn[n] = 1.0;
+          for (int k = 1; k <= lab; k++) rn[k * g->order + n] = r * rn[(k - 1) * g->order + n];
+        }
+      }
+      for (int i = nf - 1; i >= 0; i--) {
+        int l1 = fl[3 * i], l2 = fl[3 * i + 1], l3 = fl[3 * i + 2];
+        ps.Ka = Ka + l1 * g->order;
+        ps.Kb = Kb + l2 * g->order;
+        ps.rn = rn + l3 * g->order;
+        h->st.psm92_calls[1]++;
+        if (0 != orc_integrate_psm92(q2_integrand, &ps, g)) {
+          h->st.psm92_fail[1]++;
+          rc = 2;
+          break;
+        }
+        T[l1 * Tinc2 + l2 * Tinc1 + l3] += g->I;
+      }
+      orc_grid_free(g);
+    }
+  }
+  free(expo);
+  free(Ub);
+  free(Ka);
+  free(Kb);
+  free(rn);
+  return rc;
+}
+
+/* Type-2 block over C-centred monomials, C_DIM(la) x C_DIM(lb):
+   sum_l sum_{lambda1,lambda2} (sum_m OmegaA OmegaB) T_l[lambda1][lambda2][alpha+beta];
+   type2.c:532-631.  Returns NULL on failure. */
+static double *calc_gamma(orc_handle *h, const arr_t *F, const arr_t *UT, const orc_ecp *U, double dAC,
+                          double dBC, int la, int sha, int offa, const arr_t *oA, int shifta, int lb, int shb,
+                          int offb, const arr_t *oB, int shiftb) {
+  const int incA1 = oA->d[2], incA2 = oA->d[1] * oA->d[2];
+  const int incB1 = oB->d[2], incB2 = oB->d[1] * oB->d[2];
+  const int Finc1 = F->d[3], Finc2 = F->d[2] * F->d[3], Finc3 = F->d[1] * F->d[2] * F->d[3];
+  const int Uinc1 = UT->d[2], Uinc2 = UT->d[1] * UT->d[2];
+  const int inc = ODIM_C(lb);
+  double *gamma = calloc((size_t)ODIM_C(la) * ODIM_C(lb), sizeof(double));
+  orc_grid *g2 = h->grid2;
+  for (int l = 0; l < U->L; l++) {
+    const int laC = la + l, lbC = lb + l, labn = la + lb;
+    const int Tinc1 = labn + 1, Tinc2 = (lbC + 1) * (labn + 1);
+    double *T = calloc((size_t)(laC + 1) * (lbC + 1) * (labn + 1), sizeof(double));
+    int *fl = calloc(3 * (size_t)(laC + 1) * (lbC + 1) * (labn + 1), sizeof(int));
+    int nf = 0;
+    int p = 0 > shifta ? 0 : shifta, q = 0 > shiftb ? 0 : shiftb;
+    const double *Fa = F->a + p * Finc3 + sha * Finc2;
+    const double *Fb = F->a + q * Finc3 + shb * Finc2;
+    t_params tp;
+    for (int l1 = 0; l1 <= laC; l1++) {
+      tp.Fa = Fa + l1 * Finc1;
+      for (int l2 = 0; l2 <= lbC; l2++) {
+        tp.Fb = Fb + l2 * Finc1;
+        for (int l3 = 0; l3 <= labn; l3++) {
+          tp.U = UT->a + l * Uinc2 + l3 * Uinc1;
+          h->st.ps93_calls[1]++;
+          if (0 != orc_integrate_ps93(t_integrand, &tp, g2)) {
+            h->st.ps93_fail[1]++;
+            fl[3 * nf] = l1;
+            fl[3 * nf + 1] = l2;
+            fl[3 * nf + 2] = l3;
+            nf++;
+          } else {
+            T[l1 * Tinc2 + l2 * Tinc1 + l3] = g2->I;
+          }
+        }
+      }
+    }
+    if (nf > 0 && calc_t_fallback(h, T, Tinc1, Tinc2, fl, nf, U, l, dAC, la, sha, offa, shifta, dBC, lb, shb,
+                                  offb, shiftb)) {
+      free(T);
+      free(fl);
+      free(gamma);
+      return NULL;
+    }
+    free(fl);
+    for (int alpha = 0; alpha <= la; alpha++) {
+      int parity = (alpha + l) % 2;
+      int ll1 = l - alpha;
+      ll1 = (parity > ll1) ? parity : ll1;
+      for (int c1 = 0; c1 < ODIM_IJK(alpha); c1++) {
+        int pp = OIDX_C(alpha, c1);
+        for (int beta = 0; beta <= lb; beta++) {
+          int lab = alpha + beta;
+          int par2 = (beta + l) % 2;
+          int ll2 = l - beta;
+          ll2 = (par2 > ll2) ? par2 : ll2;
+          for (int c2 = 0; c2 < ODIM_IJK(beta); c2++) {
+            int qq = OIDX_C(beta, c2);
+            double tmp = 0.0;
+            for (int lam1 = ll1; lam1 <= laC; lam1 += 2)
+              for (int lam2 = ll2; lam2 <= lbC; lam2 += 2) {
+                double factor = 0.0;
+                for (int m = 0; m < 2 * l + 1; m++)
+                  factor += oA->a[lam1 * incA2 + OIDX_LM(l, m) * incA1 + OIDX_C(alpha, c1)] *
+                            oB->a[lam2 * incB2 + OIDX_LM(l, m) * incB1 + OIDX_C(beta, c2)];
+                tmp += factor * T[lam1 * Tinc2 + lam2 * Tinc1 + lab];
+              }
+            gamma[pp * inc + qq] += tmp;
+          }
+        }
+      }
+    }
+    free(T);
+  }
+  (void)g2;
+  return gamma;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Handle                                                                   */
+
+orc_handle *orc_init(int nrAtoms, const double *geometry, const int *shellsECP, const int *lECP,
+                     const int *KECP, const double *nECP, const double *dECP, const double *aECP,
+                     const int *shellsBS, const int *lBS, const int *KBS, const double *dBS,
+                     const double *aBS, int n, int lmax, const int *shellOrdering, int largeGridOrder,
+                     double tolerance, double accuracy, int options) {
+  orc_handle *h = calloc(1, sizeof(orc_handle));
+  int idx, L, N;
+  h->nrAtoms = nrAtoms;
+  h->geom = geometry;
+  h->shells = shellsBS;
+  h->am = lBS;
+  h->K = KBS;
+  h->d = dBS;
+  h->a = aBS;
+  h->deriv = n;
+  h->tolerance = tolerance;
+  h->accuracy = accuracy;
+  h->lnAcc1 = log(accuracy) - 2;
+  h->acc2 = 1.0E-14;
+  h->lnAcc2 = log(h->acc2) - 2;
+  h->options = options;
+  h->U = calloc((size_t)nrAtoms, sizeof(orc_ecp *));
+  /* ECP unpack: libecp.c:92-127 */
+  idx = 0;
+  for (int i = 0; i < nrAtoms; i++) {
+    if (shellsECP[i] > 0) {
+      L = N = 0;
+      for (int j = 0; j < shellsECP[i]; j++) {
+        if (L < lECP[idx]) L = lECP[idx];
+        N += KECP[idx];
+        idx++;
+      }
+      h->U[i] = calloc(1, sizeof(orc_ecp));
+      h->U[i]->L = L;
+      h->U[i]->N = N;
+      h->U[i]->g = calloc((size_t)N, sizeof(orc_gauss));
+      if (L > h->maxLECP) h->maxLECP = L;
+    }
+  }
+  L = N = 0;
+  for (int i = 0; i < nrAtoms; i++) {
+    if (h->U[i]) {
+      int k2 = 0;
+      for (int j = 0; j < shellsECP[i]; j++) {
+        for (int k = 0; k < KECP[L]; k++) {
+          h->U[i]->g[k2].l = lECP[L];
+          h->U[i]->g[k2].n = nECP[N];
+          h->U[i]->g[k2].d = dECP[N];
+          h->U[i]->g[k2].a = aECP[N];
+          N++;
+          k2++;
+        }
+        L++;
+      }
+    }
+  }
+  h->maxLAtom = calloc((size_t)nrAtoms, sizeof(int));
+  idx = 0;
+  for (int i = 0; i < nrAtoms; i++) {
+    if (shellsBS[i] > h->maxShells) h->maxShells = shellsBS[i];
+    for (int j = 0; j < shellsBS[i]; j++) {
+      if (h->maxLAtom[i] < lBS[idx]) h->maxLAtom[i] = lBS[idx];
+      if (h->maxLBS < lBS[idx]) h->maxLBS = lBS[idx];
+      idx++;
+    }
+  }
+  h->nrShells = idx;
+  h->maxAlpha = h->maxLBS + n;
+  h->maxLambda = h->maxLECP - 1 + h->maxAlpha;
+  h->fac = orc_fac_table(2 * (h->maxLambda + h->maxAlpha) + 1, 1);
+  h->dfac = orc_fac_table(2 * (h->maxLambda + h->maxAlpha) + 1, 2);
+  if (!h->fac || !h->dfac) {
+    orc_free(h);
+    return NULL;
+  }
+  if (NULL == shellOrdering) {
+    h->ownIJK = 1;
+    h->ijk = orc_cart_order(h->maxLambda + h->maxAlpha);
+    h->ijkIndex = orc_cart_order_index(h->maxLambda + h->maxAlpha, h->ijk);
+    h->ijkDim = h->maxLambda + h->maxAlpha + 1;
+  } else {
+    if (lmax < h->maxLambda + h->maxAlpha + 1) {
+      orc_free(h);
+      return NULL;
+    }
+    h->ijk = (int *)shellOrdering;
+    h->ijkIndex = orc_cart_order_index(lmax, h->ijk);
+    h->ijkDim = lmax + 1;
+  }
+  h->tmDim = h->maxLambda + h->maxAlpha;
+  h->cart2sph = orc_cart2sph(h->tmDim, h->ijk, h->fac);
+  h->poly2sph = orc_poly2sph(h->cart2sph, h->tmDim, h->ijk, h->dfac);
+  h->largeGrid = orc_grid_new_psm92(largeGridOrder, tolerance);
+  if (0 != orc_bessel_init(&h->bessel, h->maxLECP + h->maxAlpha + 6, 16 * 100, 200, accuracy)) {
+    h->haveBessel = 1;
+    orc_free(h);
+    return NULL;
+  }
+  h->haveBessel = 1;
+  h->grid1 = orc_grid_new_ps93(128, tolerance);
+  orc_map_kk(h->grid1);
+  h->grid2 = orc_grid_new_ps93(128, tolerance);
+  orc_map_kk(h->grid2);
+  return h;
+}
+
+void orc_free(orc_handle *h) {
+  if (!h) return;
+  if (h->U) {
+    for (int i = 0; i < h->nrAtoms; i++)
+      if (h->U[i]) {
+        free(h->U[i]->g);
+        free(h->U[i]);
+      }
+    free(h->U);
+  }
+  free(h->maxLAtom);
+  free(h->fac);
+  free(h->dfac);
+  if (h->ownIJK) free(h->ijk);
+  free(h->ijkIndex);
+  free(h->cart2sph);
+  free(h->poly2sph);
+  orc_grid_free(h->largeGrid);
+  if (h->haveBessel) orc_bessel_release(&h->bessel);
+  orc_grid_free(h->grid1);
+  orc_grid_free(h->grid2);
+  free(h);
+}
+
+void orc_get_stats(const orc_handle *h, orc_stats *out) { *out = h->st; }
+
+/* Derivative shift table; libecp.c:203-210 and 246-250. */
+static const struct {
+  int N;
+  int sa[10], sb[10];
+} SHIFTS[3] = {
+    {1, {0}, {0}},
+    {4, {+1, -1, 0, 0}, {0, 0, +1, -1}},
+    {10, {+2, -2, +1, +1, -1, -1, 0, 0, +1, 0}, {0, 0, +1, -1, +1, -1, +2, -2, 0, +1}},
+};
+
+/* Triplet driver: C -> A -> B >= A -> shells -> shift -> type 1, type 2;
+   libecp.c:213-404. */
+int orc_calculate(orc_handle *h, orc_callback cb, void *args) {
+  const int nAt = h->nrAtoms, d = h->deriv;
+  const double norm1 = 4.0 * M_PI, norm2 = norm1 * norm1;
+  int result = 0;
+  /* shell and primitive cursors keep their values across atoms as in
+     libecp.c:305-308 */
+  int sIdx1 = 0, pIdx1 = 0, sIdx2 = 0, pIdx2 = 0;
+  double *U_L = calloc((size_t)h->grid1->order, sizeof(double));
+  arr_t omega = tab_omega(h);
+  if (d < 0 || d > 2) {
+    free(U_L);
+    free(omega.a);
+    return 1;
+  }
+  for (int C = 0; C < nAt && !result; C++) {
+    orc_ecp *U = h->U[C];
+    if (!U) continue;
+    const double *rC = h->geom + 3 * C;
+    screen_t *sg = NULL;
+    int ferr = 0;
+    h->grid2->start = 0;
+    h->grid2->end = h->grid2->order - 1;
+    arr_t UT = tab_ecp(U, h->grid2, h->maxLambda, h->acc2);
+    arr_t FT = calc_f(h, rC, &sg, &ferr);
+    if (ferr) result = 2;
+    for (int s = 0; s < h->grid1->order; s++) U_L[s] = orc_eval_ecp(U, U->L, h->grid1->x[s]);
+    int sOff1 = 0, pOff1 = 0;
+    for (int A = 0; A < nAt && !result; A++) {
+      const double *rA = h->geom + 3 * A;
+      double rAC[3] = {rA[0] - rC[0], rA[1] - rC[1], rA[2] - rC[2]};
+      double dAC;
+      {
+        double x = rC[0] - rA[0], y = rC[1] - rA[1], z = rC[2] - rA[2];
+        dAC = sqrt(x * x + y * y + z * z);
+      }
+      arr_t uspA = usp_table(h->maxLAtom[A] + d, rAC);
+      arr_t omA = eval_angular(h, &omega, h->maxLAtom[A] + d, rAC);
+      int sOff2 = sOff1, pOff2 = pOff1;
+      for (int B = A; B < nAt && !result; B++) {
+        const double *rB = h->geom + 3 * B;
+        double rBC[3] = {rB[0] - rC[0], rB[1] - rC[1], rB[2] - rC[2]};
+        double dBC;
+        {
+          double x = rC[0] - rB[0], y = rC[1] - rB[1], z = rC[2] - rB[2];
+          dBC = sqrt(x * x + y * y + z * z);
+        }
+        arr_t uspB = usp_table(h->maxLAtom[B] + d, rBC);
+        arr_t omB = eval_angular(h, &omega, h->maxLAtom[B] + d, rBC);
+        sIdx1 = sOff1;
+        pIdx1 = pOff1;
+        for (int s1 = 0; s1 < h->shells[A] && !result; s1++) {
+          int la = h->am[sIdx1];
+          sIdx2 = sOff2;
+          pIdx2 = pOff2;
+          for (int s2 = 0; s2 < h->shells[B] && !result; s2++) {
+            int lb = h->am[sIdx2];
+            if ((A == B && s2 < s1) || (1 == d && A == C && B == C) || (2 == d && A == C && B == C) ||
+                sg[sIdx1].skip || sg[sIdx2].skip) {
+              pIdx2 += h->K[sIdx2];
+              sIdx2++;
+              continue;
+            }
+            int gs = sg[sIdx1].start > sg[sIdx2].start ? sg[sIdx1].start : sg[sIdx2].start;
+            int ge = sg[sIdx1].end > sg[sIdx2].end ? sg[sIdx1].end : sg[sIdx2].end;
+            h->grid1->start = h->grid2->start = gs;
+            h->grid1->end = h->grid2->end = ge;
+            if (gs < ge) h->st.triplets_evaluated++;
+            for (int s = 0; s < SHIFTS[d].N && !result; s++) {
+              int shifta = SHIFTS[d].sa[s], shiftb = SHIFTS[d].sb[s];
+              if (la < -shifta || lb < -shiftb || (1 == d && C == A && 0 != shifta) ||
+                  (1 == d && C == B && 0 != shiftb))
+                continue;
+              for (int t = 1; t <= 2 && !result; t++) {
+                if (!(gs < ge)) continue;
+                int las = la + shifta, lbs = lb + shiftb;
+                double *blk;
+                if (1 == t)
+                  blk = calc_chi(h, U_L, U, rAC, dAC, rBC, dBC, las, sIdx1, pIdx1, shifta, lbs, sIdx2, pIdx2,
+                                 shiftb);
+                else
+                  blk = calc_gamma(h, &FT, &UT, U, dAC, dBC, las, sIdx1, pIdx1, &omA, shifta, lbs, sIdx2,
+                                   pIdx2, &omB, shiftb);
+                if (!blk) {
+                  result = t;
+                  break;
+                }
+                int gcols = ODIM_C(lbs);
+                if (2 == d) {
+                  if (1 == shifta && 0 == shiftb) las = la;
+                  else if (0 == shifta && 1 == shiftb) lbs = lb;
+                }
+                double *I = shift_block(h, blk, gcols, h->accuracy, (1 == t) ? norm1 : norm2, las, &uspA, lbs,
+                                        &uspB);
+                free(blk);
+                h->st.blocks[t - 1]++;
+                if (cb) cb(A, s1, la, shifta, B, s2, lb, shiftb, C, I, args);
+                free(I);
+              }
+            }
+            pIdx2 += h->K[sIdx2];
+            sIdx2++;
+          }
+          pIdx1 += h->K[sIdx1];
+          sIdx1++;
+        }
+        sOff2 = sIdx2;
+        pOff2 = pIdx2;
+        free(omB.a);
+        free(uspB.a);
+      }
+      sOff1 = sIdx1;
+      pOff1 = pIdx1;
+      free(omA.a);
+      free(uspA.a);
+    }
+    free(UT.a);
+    free(FT.a);
+    free(sg);
+  }
+  free(omega.a);
+  free(U_L);
+  return result;
+}
+
+/* ---------------------------------------------------------------------- */
+/* One-call form; getIntegrals.c:22-95.                                    */
+
+typedef struct {
+  double *I;
+  int rowdim, dim;
+  int *aoDim;
+} gi_params;
+
+static void gi_callback(int A, int sa, int la, int shifta, int B, int sb, int lb, int shiftb, int C,
+                        double *I, void *p) {
+  gi_params *P = p;
+  const int a0 = P->aoDim[A * P->dim + sa], b0 = P->aoDim[B * P->dim + sb];
+  (void)shifta; (void)shiftb; (void)C;
+  for (int i = 0; i < ODIM_IJK(la); i++)
+    for (int j = 0; j < ODIM_IJK(lb); j++) {
+      if (a0 + i > b0 + j) continue;
+      P->I[(a0 + i) * P->rowdim + (b0 + j)] += I[i * ODIM_IJK(lb) + j];
+    }
+}
+
+int orc_getIntegrals(int nrAtoms, const double *geometry, const int *shellsECP, const int *KECP,
+                     const int *lECP, const double *nECP, const double *dECP, const double *aECP,
+                     const int *shellsBS, const int *lBS, const int *KBS, const double *dBS,
+                     const double *aBS, int largeGridOrder, double tolerance, double accuracy, int rowdim,
+                     double *I, int options) {
+  gi_params P;
+  int maxShells = 0, lstart = 0, idx = 0;
+  for (int i = 0; i < nrAtoms; i++)
+    if (shellsBS[i] > maxShells) maxShells = shellsBS[i];
+  P.dim = maxShells;
+  P.aoDim = calloc((size_t)nrAtoms * maxShells + 1, sizeof(int));
+  for (int i = 0; i < nrAtoms; i++)
+    for (int j = 0; j < shellsBS[i]; j++) {
+      P.aoDim[i * maxShells + j] = lstart;
+      lstart += ODIM_IJK(lBS[idx]);
+      idx++;
+    }
+  P.I = I;
+  P.rowdim = rowdim;
+  orc_handle *h = orc_init(nrAtoms, geometry, shellsECP, lECP, KECP, nECP, dECP, aECP, shellsBS, lBS, KBS, dBS,
+                           aBS, 0, -1, NULL, largeGridOrder, tolerance, accuracy, options);
+  if (!h) {
+    free(P.aoDim);
+    return 1;
+  }
+  int rc = orc_calculate(h, gi_callback, &P);
+  orc_free(h);
+  free(P.aoDim);
+  (void)rc; /* the reference ignores the integration status, getIntegrals.c:88 */
+  return 0;
+}
