@@ -1,0 +1,71 @@
+"""Builds the product library libecp_amd/libecp_amd.so in-tree.
+
+Host side: C11 (gcc), device side: HIP for gfx950 (hipcc).  Packed-FP32
+instructions are disabled for all device code.  Usage:
+    python -m libecp_amd.build
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libecp_amd.so")
+
+C_SOURCES = ["ecp_tables.c", "ecp_job.c", "ecp_api.c"]
+HIP_SOURCES = ["ecp_device.hip"]
+
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("ECP_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+C_FLAGS = ["-O2", "-g", "-fPIC", "-std=gnu11", "-Wall"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: " + " ".join(cmd) + "\n" + out)
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    for s in C_SOURCES:
+        jobs.append(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, s), "-o",
+                     os.path.join(BUILD, s.replace(".c", ".o"))])
+    for s in HIP_SOURCES:
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o",
+                     os.path.join(BUILD, s.replace(".hip", ".o"))])
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out.strip():
+                print(out)
+    objs = [os.path.join(BUILD, s.replace(".c", ".o")) for s in C_SOURCES] + \
+           [os.path.join(BUILD, s.replace(".hip", ".o")) for s in HIP_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs, "-lm"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(with_reference: bool = True) -> None:
+    """Compiles oracle/liboracle.so and, where the reference's sources are
+    present, oracle/_ref/ (checkers for tests and the CPU baseline)."""
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir, "oracle"])
+    if with_reference and os.path.isdir("/root/reference/src"):
+        _run(["make", "-s", "-C", odir, "ref"])
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))
+    build_oracle()

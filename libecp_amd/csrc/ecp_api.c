@@ -1,0 +1,222 @@
+/* Public C API of the MI355X ECP-integral library: the reference's libecp.h
+ * handle interface and its one-call getIntegrals() (include/libecp.h,
+ * include/getIntegrals.h), backed by the HIP pipeline in ecp_device.hip.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/libecp.h"
+#include "../../include/getIntegrals.h"
+#include "ecp_host.h"
+
+static __thread char g_err[512];
+
+static void set_last_error(const char *msg) { snprintf(g_err, sizeof(g_err), "%s", msg ? msg : ""); }
+
+const char *libecp_last_error(void) { return g_err; }
+
+struct _libECPHandle {
+  int nat, n;
+  /* private copies of the inputs (the reference borrows them, libecp.c:68-74) */
+  double *geom;
+  int *shellsECP, *lECP, *KECP;
+  double *nECP, *dECP, *aECP;
+  int *shellsBS, *lBS, *KBS;
+  double *dBS, *aBS;
+  ecp_mol mol;
+  ecp_static st;
+  ecp_dev *dev;
+  ecp_job job;
+  int job_ready;
+  double *blocks;
+  unsigned char *status;
+  size_t cap_blocks, cap_status;
+};
+
+static void *dupmem(const void *src, size_t bytes) {
+  void *p = malloc(bytes ? bytes : 1);
+  if (p && bytes) memcpy(p, src, bytes);
+  return p;
+}
+
+void libECP_free(libECPHandle *h) {
+  if (!h) return;
+  if (h->job_ready) ecp_job_free(&h->job);
+  ecp_dev_destroy(h->dev);
+  ecp_static_free(&h->st);
+  free(h->geom);
+  free(h->shellsECP);
+  free(h->lECP);
+  free(h->KECP);
+  free(h->nECP);
+  free(h->dECP);
+  free(h->aECP);
+  free(h->shellsBS);
+  free(h->lBS);
+  free(h->KBS);
+  free(h->dBS);
+  free(h->aBS);
+  free(h->blocks);
+  free(h->status);
+  free(h);
+}
+
+libECPHandle *libECP_init(int nrAtoms, double *geometry, int *shellsECP, int *lECP, int *KECP, double *nECP,
+                          double *dECP, double *aECP, int *shellsBS, int *lBS, int *KBS, double *dBS,
+                          double *aBS, int n, int lmax, int *shellOrdering, int largeGridOrder, double tolerance,
+                          double accuracy) {
+  char err[512] = "";
+  set_last_error("");
+  if (n != 0) {
+    set_last_error("derivative integrals (n > 0) are not available in this build");
+    return NULL;
+  }
+  if (nrAtoms <= 0 || !geometry || !shellsECP || !shellsBS) {
+    set_last_error("invalid arguments");
+    return NULL;
+  }
+  libECPHandle *h = calloc(1, sizeof(*h));
+  h->nat = nrAtoms;
+  h->n = n;
+  ecp_mol m0 = {nrAtoms, geometry, shellsECP, lECP, KECP, nECP, dECP, aECP, shellsBS, lBS, KBS, dBS, aBS};
+  ecp_mol_dims d;
+  if (ecp_mol_scan(&m0, &d, err, sizeof(err))) {
+    set_last_error(err);
+    free(h);
+    return NULL;
+  }
+  if (d.necp == 0) {
+    /* no ECP: the reference's tables would be built with maxLECP = 0; nothing
+       is ever integrated.  Keep a valid handle that delivers no blocks. */
+  }
+  int necps = 0;
+  for (int i = 0; i < nrAtoms; i++) necps += shellsECP[i];
+  h->geom = dupmem(geometry, sizeof(double) * 3 * (size_t)nrAtoms);
+  h->shellsECP = dupmem(shellsECP, sizeof(int) * (size_t)nrAtoms);
+  h->lECP = dupmem(lECP, sizeof(int) * (size_t)necps);
+  h->KECP = dupmem(KECP, sizeof(int) * (size_t)necps);
+  h->nECP = dupmem(nECP, sizeof(double) * (size_t)d.ngauss);
+  h->dECP = dupmem(dECP, sizeof(double) * (size_t)d.ngauss);
+  h->aECP = dupmem(aECP, sizeof(double) * (size_t)d.ngauss);
+  h->shellsBS = dupmem(shellsBS, sizeof(int) * (size_t)nrAtoms);
+  h->lBS = dupmem(lBS, sizeof(int) * (size_t)d.nsh);
+  h->KBS = dupmem(KBS, sizeof(int) * (size_t)d.nsh);
+  h->dBS = dupmem(dBS, sizeof(double) * (size_t)d.nprim);
+  h->aBS = dupmem(aBS, sizeof(double) * (size_t)d.nprim);
+  ecp_mol m = {nrAtoms, h->geom, h->shellsECP, h->lECP, h->KECP, h->nECP, h->dECP, h->aECP,
+               h->shellsBS, h->lBS, h->KBS, h->dBS, h->aBS};
+  h->mol = m;
+  const int maxL = d.maxLECP > 0 ? d.maxLECP : 1;
+  if (ecp_static_init(&h->st, maxL, d.maxLBS + n, shellOrdering, lmax, largeGridOrder, tolerance, accuracy, err,
+                      sizeof(err))) {
+    set_last_error(err);
+    libECP_free(h);
+    return NULL;
+  }
+  h->dev = ecp_dev_create(&h->st, err, sizeof(err));
+  if (!h->dev) {
+    set_last_error(err);
+    libECP_free(h);
+    return NULL;
+  }
+  return h;
+}
+
+/* Evaluates every triplet of the handle's molecule on the device and fills
+   h->blocks / h->status; returns 0, or -1 on a device/host error. */
+static int run_handle(libECPHandle *h) {
+  char err[512] = "";
+  if (h->job_ready) {
+    ecp_job_free(&h->job);
+    h->job_ready = 0;
+  }
+  if (ecp_job_build(&h->mol, 1, &h->st, &h->job, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  h->job_ready = 1;
+  if (h->job.nblk > h->cap_blocks) {
+    free(h->blocks);
+    h->blocks = malloc(sizeof(double) * h->job.nblk);
+    h->cap_blocks = h->job.nblk;
+  }
+  if (2 * (size_t)h->job.ntrip > h->cap_status) {
+    free(h->status);
+    h->status = malloc(2 * (size_t)h->job.ntrip);
+    h->cap_status = 2 * (size_t)h->job.ntrip;
+  }
+  if (h->job.ntrip == 0) return 0;
+  if (ecp_dev_run(h->dev, &h->job, h->blocks, h->status, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  return 0;
+}
+
+int calculateECPIntegrals(libECPHandle *h, ECPCallback cb, void *args) {
+  if (!h) {
+    set_last_error("NULL handle");
+    return -1;
+  }
+  if (run_handle(h)) return -1;
+  const ecp_job *J = &h->job;
+  /* callbacks in the reference order (libecp.c:372); stop at the first
+     failed integration with its documented code (libecp.h:23-26) */
+  for (int t = 0; t < J->ntrip; t++) {
+    const ecp_triplet *tr = &J->trip[t];
+    const ecp_shell *Sa = &J->shells[tr->sa], *Sb = &J->shells[tr->sb];
+    const int A = Sa->atom, B = Sb->atom, C = J->cent[tr->centre].atom;
+    const int sa = tr->sa - J->atom_shell0[A], sb = tr->sb - J->atom_shell0[B];
+    if (h->status[2 * t]) return 1;
+    if (cb) cb(A, sa, Sa->l, 0, B, sb, Sb->l, 0, C, h->blocks + tr->blk1, args);
+    if (h->status[2 * t + 1]) return 2;
+    if (cb) cb(A, sa, Sa->l, 0, B, sb, Sb->l, 0, C, h->blocks + tr->blk2, args);
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* getIntegrals (getIntegrals.c:22-95): upper-triangle += of every block. */
+
+int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *lECP, double *nECP, double *dECP,
+                 double *aECP, int *shellsBS, int *lBS, int *KBS, double *dBS, double *aBS, int largeGridOrder,
+                 double tolerance, double accuracy, int rowdim, double *I) {
+  libECPHandle *h = libECP_init(nrAtoms, geometry, shellsECP, lECP, KECP, nECP, dECP, aECP, shellsBS, lBS, KBS,
+                                dBS, aBS, 0, -1, NULL, largeGridOrder, tolerance, accuracy);
+  if (!h) {
+    printf("error initializing libECP\n");
+    return 1;
+  }
+  if (run_handle(h) == 0) {
+    const ecp_job *J = &h->job;
+    /* AO offsets: atom-major, IJK(l) per shell (getIntegrals.c:59-72) */
+    int *aoff = malloc(sizeof(int) * ((size_t)J->nsh + 1));
+    int lstart = 0;
+    for (int s = 0; s < J->nsh; s++) {
+      aoff[s] = lstart;
+      lstart += ECP_DIM_IJK(J->shells[s].l);
+    }
+    for (int t = 0; t < J->ntrip; t++) {
+      const ecp_triplet *tr = &J->trip[t];
+      const int la = J->shells[tr->sa].l, lb = J->shells[tr->sb].l;
+      const int a0 = aoff[tr->sa], b0 = aoff[tr->sb];
+      const int na = ECP_DIM_IJK(la), nb = ECP_DIM_IJK(lb);
+      for (int type = 0; type < 2; type++) {
+        if (h->status[2 * t + type]) goto done; /* reference stops at the failure */
+        const double *blk = h->blocks + (type ? tr->blk2 : tr->blk1);
+        for (int i = 0; i < na; i++)
+          for (int j = 0; j < nb; j++) {
+            if (a0 + i > b0 + j) continue;
+            I[(size_t)(a0 + i) * rowdim + (b0 + j)] += blk[i * nb + j];
+          }
+      }
+    }
+  done:
+    free(aoff);
+  }
+  libECP_free(h);
+  return 0;
+}

@@ -1,0 +1,113 @@
+/* Internal interface between the C host library and the HIP device layer. */
+#ifndef ECP_HOST_H
+#define ECP_HOST_H 1
+
+#include <stddef.h>
+#include "ecp_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- tables (ecp_tables.c) ---- */
+double *ecp_factorials(int n, int step);
+int *ecp_cart_order(int am);
+int *ecp_cart_index(int am, const int *ijk);
+int ecp_tm_offset(int l);
+double *ecp_cart2sph(int lmax, const int *xyz, const double *fac);
+double *ecp_poly2sph(const double *M, int lmax, const int *xyz, const double *fac2);
+int ecp_grid_psm92(int maxPoints, double *x, double *w);
+int ecp_grid_ps93_kk(double *x, double *w);
+void ecp_schedule_ps93(ecp_schedule *S, int *pairs_l, int *pairs_r);
+void ecp_schedule_psm92(ecp_schedule *S, int order, int *pairs_l, int *pairs_r);
+int ecp_bessel_table(double accuracy, int N, int cutoff, int rows, double *Kdev);
+double *ecp_omega(int maxL, int maxLambda, int maxAlpha, const int *ijk, const int *ijkIndex, int ijkDim,
+                  const double *cart2sph, const double *poly2sph, int tmDim, const double *dfac);
+
+/* ---- static (geometry-independent) tables shared by every job ---- */
+typedef struct {
+  double xs[ECP_SMALL_ORDER], ws[ECP_SMALL_ORDER];
+  double xl[ECP_LARGE_MAX], wl[ECP_LARGE_MAX];
+  int lorder;
+  ecp_schedule s93, s92;
+  int p93l[ECP_SMALL_ORDER], p93r[ECP_SMALL_ORDER];
+  int p92l[ECP_LARGE_MAX], p92r[ECP_LARGE_MAX];
+  double *bessel;            /* (ECP_BESSEL_N+1) x ECP_BESSEL_STRIDE */
+  double *fac, *dfac;
+  int *ijk, *ijkIndex, ijkDim;
+  int tmDim;
+  double *cart2sph, *poly2sph;
+  int omL, omLambda, omAlpha; /* dims of the Omega table */
+  double *omega;
+  double tolerance, accuracy, lnAcc1, lnAcc2;
+} ecp_static;
+
+/* ---- per-call job (flat arrays, host side) ---- */
+typedef struct {
+  int nmol;
+  int nat, nsh, nprim, ncent, ngauss;
+  ecp_xyz *atoms;            /* [nat] */
+  int *atom_maxl;            /* [nat] max basis l on the atom */
+  int *atom_shell0, *atom_nsh; /* [nat] first global shell, count */
+  ecp_shell *shells;         /* [nsh] */
+  double *pa, *pd;           /* [nprim] exponents, coefficients */
+  ecp_centre *cent;          /* [ncent] */
+  ecp_gauss *gauss;          /* [ngauss] */
+  ecp_window *win;           /* per centre, per shell of its molecule */
+  int nwin;
+  int ntrip;
+  ecp_triplet *trip;         /* [ntrip], per molecule in the reference order */
+  int *trip_mol_begin;       /* [nmol+1] */
+  size_t nblk;               /* doubles of output blocks */
+  int maxK;                  /* largest contraction length */
+  int maxL;                  /* largest ECP local label */
+  int FL;                    /* F rows per (centre, shell) slab */
+  int UK;                    /* r-power rows of the Ut table */
+  size_t nUL, nUt, nF, nang; /* table sizes (doubles) */
+  int angL;                  /* l-dimension of the per-(centre, atom) angular table */
+  int ang_lam, ang_c;        /* its lambda and Cartesian dimensions */
+} ecp_job;
+
+/* ---- device layer (ecp_device.hip) ---- */
+typedef struct ecp_dev ecp_dev;
+
+ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errlen);
+void ecp_dev_destroy(ecp_dev *d);
+/* Runs the whole pipeline for a job: returns 0 and fills blocks (job->nblk
+   doubles) and status (2 bytes per triplet: type-1 / type-2 failure flags),
+   or a negative value on a device error (message in err). */
+int ecp_dev_run(ecp_dev *d, const ecp_job *job, double *blocks, unsigned char *status, char *err,
+                size_t errlen);
+/* Timing of the last run's kernels (ms), for the benchmark. */
+int ecp_dev_last_timing(ecp_dev *d, double *ms, int n);
+
+/* ---- molecules and jobs (ecp_job.c) ---- */
+typedef struct {
+  int nat;
+  const double *geom;
+  const int *shellsECP, *lECP, *KECP;
+  const double *nECP, *dECP, *aECP;
+  const int *shellsBS, *lBS, *KBS;
+  const double *dBS, *aBS;
+} ecp_mol;
+
+typedef struct {
+  int maxLECP, maxLBS, nsh, nprim, ngauss, necp, nao;
+} ecp_mol_dims;
+
+int ecp_mol_scan(const ecp_mol *m, ecp_mol_dims *d, char *err, size_t errlen);
+int ecp_static_init(ecp_static *st, int maxL, int maxAlpha, const int *ordering, int lmaxOrdering,
+                    int largeGridOrder, double tolerance, double accuracy, char *err, size_t errlen);
+void ecp_static_free(ecp_static *st);
+int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *job, char *err,
+                  size_t errlen);
+void ecp_job_free(ecp_job *job);
+
+/* ---- host helpers (ecp_job.c) ---- */
+double ecp_eval_ecp_host(const ecp_gauss *g, int ng, int l, double r);
+double ecp_shell_radius(int K, int l, const double *d, const double *a, double zero, int *ok);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

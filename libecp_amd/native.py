@@ -1,0 +1,100 @@
+"""ctypes bindings of the product library ``libecp_amd.so``.
+
+The library exports the reference's C API (``include/libecp.h``,
+``include/getIntegrals.h``).  Loading fails loudly when the library has not
+been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libecp_amd.so")
+
+CB = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                 C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_void_p)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m libecp_amd.build` first")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.libECP_init.restype = C.c_void_p
+        _lib.libecp_last_error.restype = C.c_char_p
+    return _lib
+
+
+def last_error() -> str:
+    return lib().libecp_last_error().decode()
+
+
+def _P(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _ijk(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def get_integrals(m, grid=1024, tol=1e-12, acc=1e-14, I=None):
+    """getIntegrals() on a :class:`~libecp_amd.workloads.Molecule`; returns
+    (rc, matrix) with the upper triangle filled."""
+    L = lib()
+    nao = m.nao
+    if I is None:
+        I = np.zeros((nao, nao))
+    rc = L.getIntegrals(C.c_int(m.nat), _P(m.geometry, C.c_double), _P(m.shellsECP, C.c_int),
+                        _P(m.KECP, C.c_int), _P(m.lECP, C.c_int), _P(m.nECP, C.c_double),
+                        _P(m.dECP, C.c_double), _P(m.aECP, C.c_double), _P(m.shellsBS, C.c_int),
+                        _P(m.lBS, C.c_int), _P(m.KBS, C.c_int), _P(m.dBS, C.c_double),
+                        _P(m.aBS, C.c_double), C.c_int(grid), C.c_double(tol), C.c_double(acc),
+                        C.c_int(nao), _P(I, C.c_double))
+    return rc, I
+
+
+class Handle:
+    """libECP_init / calculateECPIntegrals / libECP_free."""
+
+    def __init__(self, m, n=0, lmax=-1, ordering=None, grid=1024, tol=1e-12, acc=1e-14):
+        L = lib()
+        self._m = m  # keep the arrays alive
+        self._ord = None if ordering is None else np.ascontiguousarray(ordering, dtype=np.int32)
+        op = None if self._ord is None else _P(self._ord, C.c_int)
+        self.h = L.libECP_init(C.c_int(m.nat), _P(m.geometry, C.c_double), _P(m.shellsECP, C.c_int),
+                               _P(m.lECP, C.c_int), _P(m.KECP, C.c_int), _P(m.nECP, C.c_double),
+                               _P(m.dECP, C.c_double), _P(m.aECP, C.c_double),
+                               _P(m.shellsBS, C.c_int), _P(m.lBS, C.c_int), _P(m.KBS, C.c_int),
+                               _P(m.dBS, C.c_double), _P(m.aBS, C.c_double), C.c_int(n),
+                               C.c_int(lmax), op, C.c_int(grid), C.c_double(tol), C.c_double(acc))
+        if not self.h:
+            raise RuntimeError("libECP_init failed: " + last_error())
+
+    def blocks(self):
+        """Runs calculateECPIntegrals and returns (rc, keys, blocks)."""
+        keys, blocks = [], []
+
+        def cb(A, sa, la, shifta, B, sb, lb, shiftb, Cc, I, args):
+            n = _ijk(la + shifta) * _ijk(lb + shiftb)
+            keys.append((A, sa, la, shifta, B, sb, lb, shiftb, Cc))
+            blocks.append(np.ctypeslib.as_array(I, shape=(n,)).copy())
+        f = CB(cb)
+        rc = lib().calculateECPIntegrals(C.c_void_p(self.h), f, None)
+        return rc, keys, blocks
+
+    def close(self):
+        if self.h:
+            lib().libECP_free(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -34,6 +34,10 @@ typedef void (*orc_callback)(int A, int sa, int la, int shifta,
 
 enum {
   ORC_OPT_CLEAN_FALLBACK = 1,
+  /* sensitivity probe: scale every type-1 small-grid integrand value by
+     (1 + 2^-50), i.e. a few ulp, to expose convergence decisions that a
+     last-bit change flips */
+  ORC_OPT_PERTURB_T1 = 2,
 };
 
 typedef struct {
@@ -42,6 +46,10 @@ typedef struct {
   long left_breaks[2];
   long blocks[2];
   long triplets_evaluated;                /* (C, a, b) with a non-empty window */
+  long fb_tabulations[2];                 /* primitive pairs tabulated on the large grid */
+  long fb_points[2];                      /* large-grid points with an evaluated integrand */
+  long ps93_level[2][16];                 /* convergence level histogram (15 = failed) */
+  long psm92_level[2][11];                /* convergence level histogram (10 = failed) */
 } orc_stats;
 
 typedef struct orc_handle orc_handle;

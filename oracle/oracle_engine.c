@@ -11,6 +11,8 @@
 
 #include "oracle_internal.h"
 
+extern int orc_last_level;
+
 struct orc_handle {
   int nrAtoms;
   const double *geom;
@@ -210,6 +212,7 @@ typedef struct {
   double C;
   const double *expo, *K, *U, *rn;
   double minExp;
+  double scale;
 } q_params;
 
 /* C r^l1 U_L(r) K_l2(s r) exp(expo); type1.c:78-88 */
@@ -218,7 +221,7 @@ static double q_integrand(int index, const void *vp) {
   double Q = 0.0;
   double e = p->expo[index];
   if (e >= p->minExp) Q = p->C * p->rn[index] * p->U[index] * p->K[index] * exp(e);
-  return Q;
+  return Q * p->scale;
 }
 
 /* Q[l1][l2], l2 <= l1 <= lab with l1 - l2 even: PS93 on the KK grid window,
@@ -239,6 +242,7 @@ static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, 
   double *rn = calloc((size_t)(lab + 1) * lo, sizeof(double));
   q_params ps;
   ps.C = ca * cb * exp(zd2);
+  ps.scale = (h->options & ORC_OPT_PERTURB_T1) ? 1.0 + ldexp(1.0, -50) : 1.0;
   ps.expo = expo;
   ps.minExp = h->lnAcc1;
   ps.U = Ub;
@@ -255,7 +259,9 @@ static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, 
     for (int l2 = l1; l2 >= 0; l2 -= 2) {
       ps.K = K + l2 * so;
       h->st.ps93_calls[0]++;
-      if (0 != orc_integrate_ps93(q_integrand, &ps, g1)) {
+      int rc93 = orc_integrate_ps93(q_integrand, &ps, g1);
+      h->st.ps93_level[0][orc_last_level]++;
+      if (0 != rc93) {
         h->st.ps93_fail[0]++;
         failed[2 * nFailed] = l1;
         failed[2 * nFailed + 1] = l2;
@@ -274,6 +280,8 @@ static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, 
       memset(Ub, 0, sizeof(double) * (size_t)lo);
     }
     ps.C = ca * cb;
+    ps.scale = 1.0;
+    h->st.fb_tabulations[0]++;
     orc_map_fm06(g, zeta_p, P);
     for (int n = 0; n < g->order; n++) {
       double r = g->x[n];
@@ -285,6 +293,7 @@ static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, 
           for (int m = n + 1; m < g->order; m++) expo[m] = -INFINITY;
         break;
       } else if (expo[n] >= ps.minExp) {
+        h->st.fb_points[0]++;
         orc_bessel_weighted(&h->bessel, g->order, lab, s * r, K + n);
         Ub[n] = orc_eval_ecp(U, U->L, r);
         rn[n] = 1.0;
@@ -296,7 +305,9 @@ static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, 
       ps.rn = rn + l1 * g->order;
       ps.K = K + l2 * g->order;
       h->st.psm92_calls[0]++;
-      if (0 != orc_integrate_psm92(q_integrand, &ps, g)) {
+      int rc92 = orc_integrate_psm92(q_integrand, &ps, g);
+      h->st.psm92_level[0][orc_last_level]++;
+      if (0 != rc92) {
         h->st.psm92_fail[0]++;
         rc = 1;
         break;
@@ -548,6 +559,7 @@ static int calc_t_fallback(orc_handle *h, double *T, int Tinc1, int Tinc2, const
       double zeta_p = zeta_a + zeta_b;
       double P = (zeta_a * dAC + zeta_b * dBC) / zeta_p;
       orc_map_fm06(g, zeta_p, P);
+      h->st.fb_tabulations[1]++;
       if (h->options & ORC_OPT_CLEAN_FALLBACK) memset(expo, 0, sizeof(double) * (size_t)lo);
       for (int n = 0; n < g->order; n++) {
         double r = g->x[n];
@@ -560,6 +572,7 @@ static int calc_t_fallback(orc_handle *h, double *T, int Tinc1, int Tinc2, const
             for (int m = n + 1; m < g->order; m++) expo[m] = -INFINITY;
           break;
         } else if (expo[n] >= ps.minExp) {
+          h->st.fb_points[1]++;
           Ub[n] = orc_eval_ecp(U, l, r);
           orc_bessel_weighted(&h->bessel, g->order, laC, s1 * r, Ka + n);
           orc_bessel_weighted(&h->bessel, g->order, lbC, s2 * r, Kb + n);
@@ -573,7 +586,9 @@ static int calc_t_fallback(orc_handle *h, double *T, int Tinc1, int Tinc2, const
         ps.Kb = Kb + l2 * g->order;
         ps.rn = rn + l3 * g->order;
         h->st.psm92_calls[1]++;
-        if (0 != orc_integrate_psm92(q2_integrand, &ps, g)) {
+        int rc92 = orc_integrate_psm92(q2_integrand, &ps, g);
+        h->st.psm92_level[1][orc_last_level]++;
+        if (0 != rc92) {
           h->st.psm92_fail[1]++;
           rc = 2;
           break;
@@ -621,7 +636,9 @@ static double *calc_gamma(orc_handle *h, const arr_t *F, const arr_t *UT, const 
         for (int l3 = 0; l3 <= labn; l3++) {
           tp.U = UT->a + l * Uinc2 + l3 * Uinc1;
           h->st.ps93_calls[1]++;
-          if (0 != orc_integrate_ps93(t_integrand, &tp, g2)) {
+          int rc93 = orc_integrate_ps93(t_integrand, &tp, g2);
+          h->st.ps93_level[1][orc_last_level]++;
+          if (0 != rc93) {
             h->st.ps93_fail[1]++;
             fl[3 * nf] = l1;
             fl[3 * nf + 1] = l2;

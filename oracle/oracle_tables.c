@@ -305,7 +305,9 @@ void orc_map_fm06(orc_grid *g, double zeta_P, double P) {
 
 /* Point-doubling integrator with the criterion 16 e^2 <= 3 N |I - q| tol;
    gc_integrators.c:38-86. */
+int orc_last_level;
 int orc_integrate_psm92(orc_integrand f, const void *ps, orc_grid *t) {
+  int level = 0;
   int order = pow(2, floor(log(t->n + 1) / log(2))) - 1;
   const double *w = t->w;
   int runs = (int)floor(log(order) / log(2));
@@ -336,12 +338,15 @@ int orc_integrate_psm92(orc_integrand f, const void *ps, orc_grid *t) {
     n = 2 * n + 1;
     N = n + 1.0;
     e = I - p;
+    level++;
     if (0 == cnt) continue;
     if (16 * e * e <= 3 * N * fabs(I - q) * t->tol) {
       t->I = 16 * I / (3 * N);
+      orc_last_level = level;
       return 0;
     }
   }
+  orc_last_level = 10;
   return 1;
 }
 
@@ -359,6 +364,7 @@ int orc_integrate_ps93(orc_integrand f, const void *ps, orc_grid *t) {
   q = w[idx] * f(idx, ps) + w[order - offset] * f(order - offset, ps);
   I = p + q;
   offset /= 2;
+  int level = 1;
   while ((2 * n * (1 - j) + j * 4 * n / 3 - 1) <= order) {
     int cnt = 0;
     j = 1 - j;
@@ -382,12 +388,15 @@ int orc_integrate_ps93(orc_integrand f, const void *ps, orc_grid *t) {
     p += (1 - j) * (I - q);
     if (0 < cnt) err = 16 * fabs((1 - j) * (q - 3 * p / 2) + j * (I - 2 * q)) / (3 * n);
     q = (1 - j) * q + j * I;
+    level++;
     if (0 == cnt) continue;
     if (err < t->tol) {
       t->I = 16 * q / (3 * n);
+      orc_last_level = level;
       return 0;
     }
   }
+  orc_last_level = 15;
   return 1;
 }
 

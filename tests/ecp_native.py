@@ -116,7 +116,9 @@ class OracleStats(C.Structure):
     _fields_ = [("ps93_calls", C.c_long * 2), ("ps93_fail", C.c_long * 2),
                 ("psm92_calls", C.c_long * 2), ("psm92_fail", C.c_long * 2),
                 ("left_breaks", C.c_long * 2), ("blocks", C.c_long * 2),
-                ("triplets_evaluated", C.c_long)]
+                ("triplets_evaluated", C.c_long), ("fb_tabulations", C.c_long * 2),
+                ("fb_points", C.c_long * 2), ("ps93_level", (C.c_long * 16) * 2),
+                ("psm92_level", (C.c_long * 11) * 2)]
 
 
 def oracle_blocks(m, n=0, grid=1024, tol=1e-12, acc=1e-14, options=0, stats=False):
