@@ -10,6 +10,7 @@
 
 #include "../../include/libecp.h"
 #include "../../include/getIntegrals.h"
+#include "../../include/libecp_ext.h"
 #include "ecp_host.h"
 
 static __thread char g_err[512];
@@ -116,13 +117,21 @@ libECPHandle *libECP_init(int nrAtoms, double *geometry, int *shellsECP, int *lE
     libECP_free(h);
     return NULL;
   }
+  return h;
+}
+
+/* The device context is created on the first calculation, so that handles
+   (and their host-side screening) exist without a GPU; the calculation
+   itself fails loudly when no HIP device is present. */
+static int ensure_device(libECPHandle *h) {
+  char err[512] = "";
+  if (h->dev) return 0;
   h->dev = ecp_dev_create(&h->st, err, sizeof(err));
   if (!h->dev) {
     set_last_error(err);
-    libECP_free(h);
-    return NULL;
+    return -1;
   }
-  return h;
+  return 0;
 }
 
 /* Evaluates every triplet of the handle's molecule on the device and fills
@@ -149,6 +158,7 @@ static int run_handle(libECPHandle *h) {
     h->cap_status = 2 * (size_t)h->job.ntrip;
   }
   if (h->job.ntrip == 0) return 0;
+  if (ensure_device(h)) return -1;
   if (ecp_dev_run(h->dev, &h->job, h->blocks, h->status, err, sizeof(err))) {
     set_last_error(err);
     return -1;
@@ -190,7 +200,12 @@ int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *
     printf("error initializing libECP\n");
     return 1;
   }
-  if (run_handle(h) == 0) {
+  if (run_handle(h)) {
+    fprintf(stderr, "libecp: %s\n", g_err);
+    libECP_free(h);
+    return 1;
+  }
+  {
     const ecp_job *J = &h->job;
     /* AO offsets: atom-major, IJK(l) per shell (getIntegrals.c:59-72) */
     int *aoff = malloc(sizeof(int) * ((size_t)J->nsh + 1));
@@ -219,4 +234,42 @@ int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *
   }
   libECP_free(h);
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Additive queries (include/libecp_ext.h).                               */
+
+int libecp_screened_triplets(libECPHandle *h, int *keys, int cap) {
+  char err[512] = "";
+  if (!h) return -1;
+  if (h->job_ready) {
+    ecp_job_free(&h->job);
+    h->job_ready = 0;
+  }
+  if (ecp_job_build(&h->mol, 1, &h->st, &h->job, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  h->job_ready = 1;
+  const ecp_job *J = &h->job;
+  for (int t = 0; t < J->ntrip && keys && t < cap; t++) {
+    const ecp_triplet *tr = &J->trip[t];
+    const int A = J->shells[tr->sa].atom, B = J->shells[tr->sb].atom;
+    keys[5 * t + 0] = J->cent[tr->centre].atom;
+    keys[5 * t + 1] = A;
+    keys[5 * t + 2] = tr->sa - J->atom_shell0[A];
+    keys[5 * t + 3] = B;
+    keys[5 * t + 4] = tr->sb - J->atom_shell0[B];
+  }
+  return J->ntrip;
+}
+
+long long libecp_nominal_triplets(libECPHandle *h) {
+  if (!h) return -1;
+  long long ne = 0, ns = 0;
+  for (int i = 0; i < h->nat; i++) {
+    ne += h->shellsECP[i] > 0;
+    ns += h->shellsBS[i];
+  }
+  return ne * ns * (ns + 1) / 2;
 }

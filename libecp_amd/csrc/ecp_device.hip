@@ -1122,7 +1122,7 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
   const size_t nom = (size_t)ECP_DIM_L(st->omL) * ECP_DIM_L(st->omLambda) * ECP_DIM_C(st->omAlpha);
   const size_t nijk = 3 * (size_t)ECP_DIM_C(st->ijkDim - 1);
   const size_t nidx = (size_t)st->ijkDim * st->ijkDim * st->ijkDim;
-  const size_t nfac = 2 * (size_t)st->tmDim + 2;
+  const size_t nfac = 2 * (size_t)st->tmDim + 6;
   std::vector<double> binom(16 * 16, 0.0);
   for (int n = 0; n < 16; n++)
     for (int k = 0; k <= n; k++) {

@@ -135,8 +135,10 @@ int ecp_static_init(ecp_static *st, int maxL, int maxAlpha, const int *ordering,
   st->accuracy = accuracy;
   st->lnAcc1 = log(accuracy) - 2;
   st->lnAcc2 = log(1.0E-14) - 2;
-  st->fac = ecp_factorials(2 * tmDim + 1, 1);
-  st->dfac = ecp_factorials(2 * tmDim + 1, 2);
+  /* at least 0..5: the reference's factorials(1) leaves 1! = 0 and its
+     TM_DIM reads fac[3] when tmDim = 0 (util.c:65-70, transformations.h:13) */
+  st->fac = ecp_factorials(2 * tmDim + 5, 1);
+  st->dfac = ecp_factorials(2 * tmDim + 5, 2);
   if (!st->fac || !st->dfac) {
     set_err(err, errlen, "factorial table overflow");
     ecp_static_free(st);

@@ -84,7 +84,8 @@ int orc_bessel_table(int lMax, int N, int cutoff, double accuracy, double *K /* 
 void orc_bessel_eval(int lMax, double accuracy, int lmax, double z, double *K /* lmax+1 */);
 void orc_real_sph(int lmax, double theta, double phi, double *out /* (lmax+1)^2 */);
 void orc_spherical_coordinates(const double *xyz, double *rtp);
-/* Full type-1 / type-2 block for one triplet through a handle. */
+double orc_quad_ps93(double (*f)(double), double tol, int *rc);
+double orc_quad_psm92(int maxPoints, double zeta_p, double P, double (*f)(double), double tol, int *rc);
 double orc_shell_radius(int K, int l, const double *d, const double *a, double zero);
 
 #ifdef __cplusplus

@@ -705,3 +705,36 @@ double orc_shell_radius(int K, int l, const double *d, const double *a, double z
   int ok;
   return orc_shell_radius_(K, l, d, a, zero, &ok);
 }
+
+/* Quadrature hooks for the unit tests: f(r) integrated by the PS93
+   sequence on the Krack-Koster grid, or by PSM92 on the FM06-mapped large
+   grid (zeta_p, P), with the integrators' own convergence logic. */
+typedef struct {
+  double (*f)(double);
+  const double *x;
+} orc_hook;
+
+static double hook_integrand(int index, const void *p) {
+  const orc_hook *h = p;
+  return h->f(h->x[index]);
+}
+
+double orc_quad_ps93(double (*f)(double), double tol, int *rc) {
+  orc_grid *g = orc_grid_new_ps93(128, tol);
+  orc_map_kk(g);
+  orc_hook h = {f, g->x};
+  *rc = orc_integrate_ps93(hook_integrand, &h, g);
+  double I = g->I;
+  orc_grid_free(g);
+  return I;
+}
+
+double orc_quad_psm92(int maxPoints, double zeta_p, double P, double (*f)(double), double tol, int *rc) {
+  orc_grid *g = orc_grid_new_psm92(maxPoints, tol);
+  orc_map_fm06(g, zeta_p, P);
+  orc_hook h = {f, g->x};
+  *rc = orc_integrate_psm92(hook_integrand, &h, g);
+  double I = g->I;
+  orc_grid_free(g);
+  return I;
+}

@@ -67,17 +67,29 @@ def _gi_args(m):
             _P(m.KBS, C.c_int), _P(m.dBS, C.c_double), _P(m.aBS, C.c_double)]
 
 
+def block_size(la, shifta, lb, shiftb, n):
+    """Elements of one callback block: IJK(la+shifta) x IJK(lb+shiftb), except
+    that for n = 2 the (1,0) and (0,1) shifts use the unshifted l
+    (libecp.c:364-369)."""
+    las, lbs = la + shifta, lb + shiftb
+    if n == 2 and (shifta, shiftb) == (1, 0):
+        las = la
+    elif n == 2 and (shifta, shiftb) == (0, 1):
+        lbs = lb
+    return ijk_dim(las) * ijk_dim(lbs)
+
+
 class Blocks:
     """Collects the callback stream (libecp.c:372 argument order)."""
 
-    def __init__(self):
+    def __init__(self, n=0):
         self.keys = []
         self.blocks = []
 
         def cb(A, sa, la, shifta, B, sb, lb, shiftb, Cc, I, args):
-            n = ijk_dim(la + shifta) * ijk_dim(lb + shiftb)
+            nn = block_size(la, shifta, lb, shiftb, n)
             self.keys.append((A, sa, la, shifta, B, sb, lb, shiftb, Cc))
-            self.blocks.append(np.ctypeslib.as_array(I, shape=(n,)).copy())
+            self.blocks.append(np.ctypeslib.as_array(I, shape=(nn,)).copy())
         self.cb = CB(cb)
 
 
@@ -97,7 +109,7 @@ def ref_blocks(m, n=0, grid=1024, tol=1e-12, acc=1e-14, lib=None):
                         C.c_int(grid), C.c_double(tol), C.c_double(acc))
     if not h:
         return None, None
-    col = Blocks()
+    col = Blocks(n)
     rc = lib.calculateECPIntegrals(C.c_void_p(h), col.cb, None)
     lib.libECP_free(C.c_void_p(h))
     return rc, col
@@ -128,7 +140,7 @@ def oracle_blocks(m, n=0, grid=1024, tol=1e-12, acc=1e-14, options=0, stats=Fals
                      C.c_int(grid), C.c_double(tol), C.c_double(acc), C.c_int(options))
     if not h:
         return (None, None, None) if stats else (None, None)
-    col = Blocks()
+    col = Blocks(n)
     rc = lib.orc_calculate(C.c_void_p(h), col.cb, None)
     st = OracleStats()
     lib.orc_get_stats(C.c_void_p(h), C.byref(st))
