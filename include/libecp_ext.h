@@ -22,6 +22,42 @@ int libecp_screened_triplets(libECPHandle *h, int *keys, int cap);
    screening (libecp.c:256-312), the unit of the throughput metric. */
 long long libecp_nominal_triplets(libECPHandle *h);
 
+/* ---- batched evaluation ------------------------------------------------
+   Many molecules in one resident job: one upload, one kernel sequence over
+   the triplets of every molecule (replayed from a captured HIP graph), one
+   download.  Molecule arrays are copied at creation. */
+
+typedef struct {
+  int nrAtoms;
+  const double *geometry;
+  const int *shellsECP, *KECP, *lECP;   /* getIntegrals order: KECP before lECP */
+  const double *nECP, *dECP, *aECP;
+  const int *shellsBS, *lBS, *KBS;
+  const double *dBS, *aBS;
+} libecp_molecule;
+
+typedef struct libecp_batch libecp_batch;
+
+/* Screens every molecule on the host and uploads the job.  NULL on error
+   (libecp_last_error()). */
+libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int largeGridOrder, double tolerance,
+                                  double accuracy);
+/* Evaluates the whole batch `repeats` times on the device and waits;
+   *ms = device time of one evaluation.  0 on success, -1 on a device error. */
+int libecp_batch_execute(libecp_batch *b, int repeats, double *ms);
+/* Copies the results of the last evaluation to the host. */
+int libecp_batch_fetch(libecp_batch *b);
+/* Adds molecule `mol`'s ECP matrix into the upper triangle of I (row stride
+   rowdim), as getIntegrals does; returns 0, or 1 / 2 when a type-1 / type-2
+   integration of that molecule failed (blocks up to the failure added). */
+int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I);
+/* Nominal (pre-screening) and evaluated triplet counts of the batch. */
+long long libecp_batch_triplets(libecp_batch *b, long long *evaluated);
+/* Stage timings (ms) of the first evaluation: tables, quadratures (small
+   grid and large-grid fallbacks), assembly; returns the count written. */
+int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n);
+void libecp_batch_free(libecp_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

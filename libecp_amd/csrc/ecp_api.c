@@ -273,3 +273,170 @@ long long libecp_nominal_triplets(libECPHandle *h) {
   }
   return ne * ns * (ns + 1) / 2;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Batched evaluation (include/libecp_ext.h).                              */
+
+struct libecp_batch {
+  int nmol;
+  ecp_mol *mols;
+  void **owned;
+  int nowned;
+  ecp_static st;
+  ecp_dev *dev;
+  ecp_job job;
+  double *blocks;
+  unsigned char *status;
+  long long nominal;
+};
+
+void libecp_batch_free(libecp_batch *b) {
+  if (!b) return;
+  ecp_dev_destroy(b->dev);
+  ecp_job_free(&b->job);
+  ecp_static_free(&b->st);
+  for (int i = 0; i < b->nowned; i++) free(b->owned[i]);
+  free(b->owned);
+  free(b->mols);
+  free(b->blocks);
+  free(b->status);
+  free(b);
+}
+
+libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int largeGridOrder, double tolerance,
+                                  double accuracy) {
+  char err[512] = "";
+  set_last_error("");
+  if (nmol <= 0 || !mols) {
+    set_last_error("empty batch");
+    return NULL;
+  }
+  libecp_batch *b = calloc(1, sizeof(*b));
+  b->nmol = nmol;
+  b->mols = calloc((size_t)nmol, sizeof(ecp_mol));
+  b->owned = calloc((size_t)nmol * 12, sizeof(void *));
+  int maxL = 1, maxLBS = 0;
+  for (int i = 0; i < nmol; i++) {
+    const libecp_molecule *M = &mols[i];
+    ecp_mol m0 = {M->nrAtoms, M->geometry, M->shellsECP, M->lECP, M->KECP, M->nECP, M->dECP, M->aECP,
+                  M->shellsBS, M->lBS, M->KBS, M->dBS, M->aBS};
+    ecp_mol_dims d;
+    if (ecp_mol_scan(&m0, &d, err, sizeof(err))) {
+      char msg[600];
+      snprintf(msg, sizeof(msg), "molecule %d: %s", i, err);
+      set_last_error(msg);
+      libecp_batch_free(b);
+      return NULL;
+    }
+    int necps = 0, ns = 0;
+    for (int a = 0; a < M->nrAtoms; a++) {
+      necps += M->shellsECP[a];
+      ns += M->shellsBS[a];
+      b->nominal += 0;
+    }
+    long long ne = d.necp;
+    b->nominal += ne * ns * (ns + 1) / 2;
+    void **o = b->owned + b->nowned;
+    o[0] = dupmem(M->geometry, sizeof(double) * 3 * (size_t)M->nrAtoms);
+    o[1] = dupmem(M->shellsECP, sizeof(int) * (size_t)M->nrAtoms);
+    o[2] = dupmem(M->lECP, sizeof(int) * (size_t)necps);
+    o[3] = dupmem(M->KECP, sizeof(int) * (size_t)necps);
+    o[4] = dupmem(M->nECP, sizeof(double) * (size_t)d.ngauss);
+    o[5] = dupmem(M->dECP, sizeof(double) * (size_t)d.ngauss);
+    o[6] = dupmem(M->aECP, sizeof(double) * (size_t)d.ngauss);
+    o[7] = dupmem(M->shellsBS, sizeof(int) * (size_t)M->nrAtoms);
+    o[8] = dupmem(M->lBS, sizeof(int) * (size_t)d.nsh);
+    o[9] = dupmem(M->KBS, sizeof(int) * (size_t)d.nsh);
+    o[10] = dupmem(M->dBS, sizeof(double) * (size_t)d.nprim);
+    o[11] = dupmem(M->aBS, sizeof(double) * (size_t)d.nprim);
+    b->nowned += 12;
+    ecp_mol m = {M->nrAtoms, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9], o[10], o[11]};
+    b->mols[i] = m;
+    if (d.maxLECP > maxL) maxL = d.maxLECP;
+    if (d.maxLBS > maxLBS) maxLBS = d.maxLBS;
+  }
+  if (ecp_static_init(&b->st, maxL, maxLBS, NULL, -1, largeGridOrder, tolerance, accuracy, err, sizeof(err)) ||
+      ecp_job_build(b->mols, nmol, &b->st, &b->job, err, sizeof(err))) {
+    set_last_error(err);
+    libecp_batch_free(b);
+    return NULL;
+  }
+  b->blocks = malloc(sizeof(double) * (b->job.nblk + 1));
+  b->status = calloc(2 * (size_t)b->job.ntrip + 1, 1);
+  b->dev = ecp_dev_create(&b->st, err, sizeof(err));
+  if (!b->dev || ecp_dev_prepare(b->dev, &b->job, err, sizeof(err))) {
+    set_last_error(err);
+    libecp_batch_free(b);
+    return NULL;
+  }
+  return b;
+}
+
+int libecp_batch_execute(libecp_batch *b, int repeats, double *ms) {
+  char err[512] = "";
+  if (!b) return -1;
+  if (ecp_dev_execute(b->dev, repeats, ms, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  return 0;
+}
+
+int libecp_batch_fetch(libecp_batch *b) {
+  char err[512] = "";
+  if (!b) return -1;
+  if (ecp_dev_fetch(b->dev, b->blocks, b->status, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  return 0;
+}
+
+int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I) {
+  if (!b || mol < 0 || mol >= b->nmol) return -1;
+  const ecp_job *J = &b->job;
+  /* AO offsets of the molecule's shells */
+  int s0 = 0;
+  for (int m = 0; m < mol; m++) {
+    const ecp_mol *M = &b->mols[m];
+    for (int a = 0; a < M->nat; a++) s0 += M->shellsBS[a];
+  }
+  int nsh = 0;
+  for (int a = 0; a < b->mols[mol].nat; a++) nsh += b->mols[mol].shellsBS[a];
+  int *aoff = malloc(sizeof(int) * ((size_t)nsh + 1));
+  int lstart = 0;
+  for (int s = 0; s < nsh; s++) {
+    aoff[s] = lstart;
+    lstart += ECP_DIM_IJK(J->shells[s0 + s].l);
+  }
+  int rc = 0;
+  for (int t = J->trip_mol_begin[mol]; t < J->trip_mol_begin[mol + 1] && !rc; t++) {
+    const ecp_triplet *tr = &J->trip[t];
+    const int la = J->shells[tr->sa].l, lb = J->shells[tr->sb].l;
+    const int a0 = aoff[tr->sa - s0], b0 = aoff[tr->sb - s0];
+    const int na = ECP_DIM_IJK(la), nb = ECP_DIM_IJK(lb);
+    for (int type = 0; type < 2; type++) {
+      if (b->status[2 * t + type]) {
+        rc = type + 1;
+        break;
+      }
+      const double *blk = b->blocks + (type ? tr->blk2 : tr->blk1);
+      for (int i = 0; i < na; i++)
+        for (int j = 0; j < nb; j++)
+          if (a0 + i <= b0 + j) I[(size_t)(a0 + i) * rowdim + (b0 + j)] += blk[i * nb + j];
+    }
+  }
+  free(aoff);
+  return rc;
+}
+
+long long libecp_batch_triplets(libecp_batch *b, long long *evaluated) {
+  if (!b) return -1;
+  if (evaluated) *evaluated = b->job.ntrip;
+  return b->nominal;
+}
+
+int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n) {
+  if (!b || !b->dev) return 0;
+  return ecp_dev_last_timing(b->dev, ms, n);
+}

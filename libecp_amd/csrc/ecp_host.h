@@ -78,7 +78,14 @@ void ecp_dev_destroy(ecp_dev *d);
    or a negative value on a device error (message in err). */
 int ecp_dev_run(ecp_dev *d, const ecp_job *job, double *blocks, unsigned char *status, char *err,
                 size_t errlen);
-/* Timing of the last run's kernels (ms), for the benchmark. */
+/* The same in three steps: upload a job (inputs become resident in HBM),
+   execute its kernel sequence `repeats` times (graph replays after the
+   first; *ms_total = device time per repeat), and copy the results back. */
+int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t errlen);
+int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *err, size_t errlen);
+int ecp_dev_fetch(ecp_dev *d, double *blocks, unsigned char *status, char *err, size_t errlen);
+/* Stage timings of the last execute (ms): tables, quadratures (small grid
+   and large-grid fallbacks, concurrent), assembly[, mean graph replay]. */
 int ecp_dev_last_timing(ecp_dev *d, double *ms, int n);
 
 /* ---- molecules and jobs (ecp_job.c) ---- */

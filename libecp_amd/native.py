@@ -98,3 +98,71 @@ class Handle:
             self.close()
         except Exception:
             pass
+
+
+class _Mol(C.Structure):
+    _fields_ = [("nrAtoms", C.c_int), ("geometry", C.POINTER(C.c_double)),
+                ("shellsECP", C.POINTER(C.c_int)), ("KECP", C.POINTER(C.c_int)), ("lECP", C.POINTER(C.c_int)),
+                ("nECP", C.POINTER(C.c_double)), ("dECP", C.POINTER(C.c_double)), ("aECP", C.POINTER(C.c_double)),
+                ("shellsBS", C.POINTER(C.c_int)), ("lBS", C.POINTER(C.c_int)), ("KBS", C.POINTER(C.c_int)),
+                ("dBS", C.POINTER(C.c_double)), ("aBS", C.POINTER(C.c_double))]
+
+
+def _mol_struct(m):
+    return _Mol(m.nat, _P(m.geometry, C.c_double), _P(m.shellsECP, C.c_int), _P(m.KECP, C.c_int),
+                _P(m.lECP, C.c_int), _P(m.nECP, C.c_double), _P(m.dECP, C.c_double), _P(m.aECP, C.c_double),
+                _P(m.shellsBS, C.c_int), _P(m.lBS, C.c_int), _P(m.KBS, C.c_int), _P(m.dBS, C.c_double),
+                _P(m.aBS, C.c_double))
+
+
+class Batch:
+    """libecp_batch_*: many molecules in one resident device job."""
+
+    def __init__(self, mols, grid=1024, tol=1e-12, acc=1e-14):
+        L = lib()
+        L.libecp_batch_create.restype = C.c_void_p
+        L.libecp_batch_triplets.restype = C.c_longlong
+        self.mols = list(mols)
+        arr = (_Mol * len(self.mols))(*[_mol_struct(m) for m in self.mols])
+        self.b = L.libecp_batch_create(arr, C.c_int(len(self.mols)), C.c_int(grid), C.c_double(tol),
+                                       C.c_double(acc))
+        if not self.b:
+            raise RuntimeError("libecp_batch_create failed: " + last_error())
+
+    def execute(self, repeats=1):
+        ms = C.c_double()
+        if lib().libecp_batch_execute(C.c_void_p(self.b), C.c_int(repeats), C.byref(ms)) != 0:
+            raise RuntimeError("libecp_batch_execute failed: " + last_error())
+        return ms.value
+
+    def fetch(self):
+        if lib().libecp_batch_fetch(C.c_void_p(self.b)) != 0:
+            raise RuntimeError("libecp_batch_fetch failed: " + last_error())
+
+    def matrix(self, i, I=None):
+        m = self.mols[i]
+        if I is None:
+            I = np.zeros((m.nao, m.nao))
+        rc = lib().libecp_batch_matrix(C.c_void_p(self.b), C.c_int(i), C.c_int(m.nao), _P(I, C.c_double))
+        return rc, I
+
+    def triplets(self):
+        ev = C.c_longlong()
+        nom = lib().libecp_batch_triplets(C.c_void_p(self.b), C.byref(ev))
+        return nom, ev.value
+
+    def stage_ms(self):
+        out = (C.c_double * 8)()
+        n = lib().libecp_batch_stage_ms(C.c_void_p(self.b), out, C.c_int(8))
+        return list(out[:n])
+
+    def close(self):
+        if self.b:
+            lib().libecp_batch_free(C.c_void_p(self.b))
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
