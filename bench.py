@@ -35,6 +35,19 @@ from libecp_amd import native  # noqa: E402
 from libecp_amd.workloads import medium_cluster, uneven_batch  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_medium_cluster.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_type1 launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a copy of known size)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if "traffic_bytes_per_launch:" in line:
+                return float(line.split("traffic_bytes_per_launch:")[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def ecp_subset(m, keep):
@@ -113,17 +126,28 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one device copy of 256 MiB (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build evaluates on one GPU; --gpus {args.gpus} is not supported "
               "(sharding by ECP centre across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     m = medium_cluster(0)
     b = native.Batch([m])
     nominal, evaluated = b.triplets()
     b.execute(max(1, args.warmup))
     stages = b.stage_ms()
+    prof = b.profile()
     t0 = time.perf_counter()
     dev_ms = b.execute(args.steps)
     wall = time.perf_counter() - t0
@@ -156,9 +180,29 @@ def main():
             "largeGridOrder": 1024, "tolerance": 1e-12, "accuracy": 1e-14,
         },
         "device_ms_per_step": dev_ms,
+        "kernels": {k: {"launches": v[0], "ms": v[1], "algorithmic_gflop": v[2] / 1e9} for k, v in prof.items()},
         "stage_ms_first_step": dict(zip(["tables", "quadratures", "assembly"], stages[:3])),
         "status": rc,
     }
+
+    # Roofline of the dominant kernel with an algorithmic model: its FLOPs per
+    # evaluation under the reference algorithm's per-unit counts (type 1:
+    # G (31 lambda + 74 + 4 n_Q) + 150 n_Q per primitive pair; type 2:
+    # 4 G + 150 per T entry) over its summed launch durations, both measured
+    # above with HIP events on the launches' own streams.
+    fam = max((k for k in ("type1", "type2") if prof[k][0] > 0), key=lambda k: prof[k][1])
+    launches, kms, kflops = prof[fam]
+    achieved = kflops / (kms * 1e-3) / 1e12
+    out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": achieved / FP64_PEAK_TFLOPS,
+                       "traffic": measured_traffic() if fam == "type1" else None,
+                       "kernel": f"k_{fam}", "launches_per_step": launches,
+                       "avg_launch_ms": kms / launches, "algorithmic_flop_per_launch": kflops / launches,
+                       "note": "FP64 vector work (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); "
+                               "algorithmic FLOPs count every window point as the reference tabulates them, "
+                               "the kernel evaluates points lazily up to convergence"}
+    step_flops = prof["type1"][2] + prof["type2"][2]
+    out["algorithmic_tflops_per_step"] = step_flops / (ms_per_step * 1e-3) / 1e12
 
     if not args.no_plain:
         times = []

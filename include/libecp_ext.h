@@ -56,6 +56,13 @@ long long libecp_batch_triplets(libecp_batch *b, long long *evaluated);
 /* Stage timings (ms) of the first evaluation: tables, quadratures (small
    grid and large-grid fallbacks), assembly; returns the count written. */
 int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n);
+/* One evaluation timed kernel by kernel (HIP events on each launch's
+   stream).  For up to n kernel families (names: libecp_kernel_family(i)):
+   number of launches, summed duration (ms), algorithmic FLOPs of one
+   evaluation under the reference algorithm's per-unit models (0 where no
+   model is defined).  Returns the number of families, or -1. */
+int libecp_batch_profile(libecp_batch *b, int *launches, double *ms, double *flops, int n);
+const char *libecp_kernel_family(int i);
 void libecp_batch_free(libecp_batch *b);
 
 #ifdef __cplusplus

@@ -440,3 +440,23 @@ int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n) {
   if (!b || !b->dev) return 0;
   return ecp_dev_last_timing(b->dev, ms, n);
 }
+
+int libecp_batch_profile(libecp_batch *b, int *launches, double *ms, double *flops, int n) {
+  char err[512] = "";
+  int l[16];
+  double m[16], f[16];
+  if (!b) return -1;
+  int k = ecp_dev_profile(b->dev, l, m, f, err, sizeof(err));
+  if (k < 0) {
+    set_last_error(err);
+    return -1;
+  }
+  for (int i = 0; i < k && i < n; i++) {
+    launches[i] = l[i];
+    ms[i] = m[i];
+    flops[i] = f[i];
+  }
+  return k;
+}
+
+const char *libecp_kernel_family(int i) { return ecp_dev_family_name(i); }

@@ -1150,6 +1150,10 @@ struct Prepared {
   size_t zero_words = 0;
   double *blk = nullptr;
   size_t asm_lds = 0;
+  // algorithmic FLOPs per evaluation (the per-unit models of the reference's
+  // eager algorithm): type-1 radial work per class, type-2 radial work
+  double flops_t1[ECP_MAX_LAB + 1] = {0};
+  double flops_t2 = 0;
 };
 
 }  // namespace
@@ -1178,7 +1182,42 @@ struct ecp_dev {
   bool prepared = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // per-launch profiling (eager runs only)
+  bool profiling = false;
+  std::vector<hipEvent_t> pev;
+  std::vector<int> pfam;
+  int npev = 0;
 };
+
+namespace {
+// kernel families reported by the profiler
+enum { FAM_TABLES = 0, FAM_TYPE1, FAM_TYPE2, FAM_FB1, FAM_FB2, FAM_ASM, FAM_N };
+const char *kFamName[FAM_N] = {"tables", "type1", "type2", "fallback1", "fallback2", "assemble"};
+
+// records an event pair around one launch on stream s when profiling
+struct Span {
+  ecp_dev *d;
+  hipStream_t s;
+  int i;
+  Span(ecp_dev *d_, hipStream_t s_, int fam) : d(d_), s(s_), i(-1) {
+    if (!d->profiling) return;
+    if ((size_t)(2 * d->npev + 2) > d->pev.size()) {
+      hipEvent_t a, b;
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      d->pev.push_back(a);
+      d->pev.push_back(b);
+      d->pfam.push_back(fam);
+    }
+    i = d->npev++;
+    d->pfam[i] = fam;
+    (void)hipEventRecord(d->pev[2 * i], s);
+  }
+  ~Span() {
+    if (i >= 0) (void)hipEventRecord(d->pev[2 * i + 1], s);
+  }
+};
+}  // namespace
 
 namespace {
 
@@ -1347,6 +1386,7 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   }
   if (d->fork) (void)hipEventDestroy(d->fork);
   if (d->t2done) (void)hipEventDestroy(d->t2done);
+  for (auto e : d->pev) (void)hipEventDestroy(e);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -1360,25 +1400,30 @@ extern "C" int ecp_dev_last_timing(ecp_dev *d, double *ms, int n) {
 namespace {
 
 template <int LAB>
-void launch_type1(hipStream_t s, const DevTables &T, const DevJob &J, const T1Unit *u, int n, double *q,
+void launch_type1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T1Unit *u, int n, double *q,
                   unsigned long long *qf, int *fbq, int *fbc) {
-  if (n > 0) k_type1<LAB><<<(n + 63) / 64, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc);
+  if (n > 0) {
+    Span sp(d, s, FAM_TYPE1);
+    k_type1<LAB><<<(n + 63) / 64, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc);
+  }
 }
 
 template <int LAB>
-void launch_fb1(hipStream_t s, const DevTables &T, const DevJob &J, const T1Unit *u, int n, double *q,
+void launch_fb1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T1Unit *u, int n, double *q,
                 const unsigned long long *qf, const int *fbq, const int *fbc, unsigned char *st) {
   if (n > 0) {
+    Span sp(d, s, FAM_FB1);
     const int blocks = std::min(n, 4096);
     k_fb1<LAB><<<blocks, 64, 0, s>>>(T, J, u, q, qf, fbq, fbc, st);
   }
 }
 
 template <int LMB>
-void launch_fb2(hipStream_t s, const DevTables &T, const DevJob &J, const T2Group *g, const int *tm,
+void launch_fb2(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T2Group *g, const int *tm,
                 const unsigned char *tf, const int *fbq, const int *fbc, double *fbT, unsigned char *st,
                 long maxItems) {
   if (maxItems > 0) {
+    Span sp(d, s, FAM_FB2);
     const long blocks = std::min<long>(maxItems, 8192);
     k_fb2<LMB><<<(int)blocks, 64, 0, s>>>(T, J, g, tm, tf, fbq, fbc, fbT, st);
   }
@@ -1397,32 +1442,37 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   const DevJob &J = P.J;
   hipStream_t *w = d->workers;
   if (evs) (void)hipEventRecord(evs[0], s);
-  k_zero<<<(int)std::min<size_t>((P.zero_words + 255) / 256, 1024), 256, 0, s>>>(P.zero_base, P.zero_words);
-  k_utab<<<P.ncent, 384, 0, s>>>(T, J);
-  if (P.nfit) k_ftab<<<P.nfit, 128, 0, s>>>(T, J, P.fit);
-  if (P.nait) k_ang<<<P.nait, 128, 0, s>>>(T, J, P.ait, d->fac_dev, d->dfac_dev);
+  {
+    Span sp(d, s, FAM_TABLES);
+    k_zero<<<(int)std::min<size_t>((P.zero_words + 255) / 256, 1024), 256, 0, s>>>(P.zero_base, P.zero_words);
+    k_utab<<<P.ncent, 384, 0, s>>>(T, J);
+    if (P.nfit) k_ftab<<<P.nfit, 128, 0, s>>>(T, J, P.fit);
+    if (P.nait) k_ang<<<P.nait, 128, 0, s>>>(T, J, P.ait, d->fac_dev, d->dfac_dev);
+  }
   if (evs) (void)hipEventRecord(evs[1], s);
   (void)hipEventRecord(d->fork, s);
   for (int i = 0; i < kWorkers; i++) (void)hipStreamWaitEvent(w[i], d->fork, 0);
   // type 2 first on the last worker: its fallbacks wait for it
-  if (P.ntot2 > 0)
+  if (P.ntot2 > 0) {
+    Span sp(d, w[kWorkers - 1], FAM_TYPE2);
     k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w[kWorkers - 1]>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev,
                                                                   P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups);
+  }
   (void)hipEventRecord(d->t2done, w[kWorkers - 1]);
   // type 1 classes, each followed by its fallback, round-robin over workers
 #define T1(c)                                                                                                   \
   {                                                                                                             \
     hipStream_t ws = w[(c) % kWorkers];                                                                         \
-    launch_type1<c>(ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                \
+    launch_type1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                \
                     P.fbq1 + P.cls_begin[c], P.fbc + c);                                                        \
-    launch_fb1<c>(ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                  \
+    launch_fb1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                  \
                   P.fbq1 + P.cls_begin[c], P.fbc + c, P.stat);                                                  \
   }
   T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10)
 #undef T1
   for (int i = 0; i < kWorkers - 1; i++) (void)hipStreamWaitEvent(w[i], d->t2done, 0);
 #define FB2(c)                                                                                                  \
-  launch_fb2<c>(w[(c) % kWorkers], T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.ngroups,            \
+  launch_fb2<c>(d, w[(c) % kWorkers], T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.ngroups,            \
                 P.fbc + 16 + (c), P.fbT, P.stat, (long)P.lmb_count[c] * P.maxPairs);
   FB2(0) FB2(1) FB2(2) FB2(3) FB2(4) FB2(5) FB2(6) FB2(7) FB2(8) FB2(9) FB2(10) FB2(11)
 #undef FB2
@@ -1431,7 +1481,10 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     (void)hipStreamWaitEvent(s, d->join[i], 0);
   }
   if (evs) (void)hipEventRecord(evs[2], s);
-  k_assemble<<<P.nt, kAsmThreads, P.asm_lds, s>>>(T, J, P.aa, P.stat);
+  {
+    Span sp(d, s, FAM_ASM);
+    k_assemble<<<P.nt, kAsmThreads, P.asm_lds, s>>>(T, J, P.aa, P.stat);
+  }
   if (evs) (void)hipEventRecord(evs[3], s);
 }
 
@@ -1470,6 +1523,9 @@ extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t
         const int u = cls_fill[c]++;
         units1[u] = T1Unit{t, pa, pb, (int)(cls_qbase[c] + (size_t)(u - P.cls_begin[c]) * nq_of(c))};
       }
+    // per primitive pair: G (31 lambda + 74 + 4 n_Q) + 150 n_Q
+    const double G = tr.ge - tr.gs;
+    P.flops_t1[c] += (double)Sa.K * Sb.K * (G * (31.0 * c + 74 + 4.0 * nq_of(c)) + 150.0 * nq_of(c));
   }
   const size_t nq = cls_qbase[ECP_MAX_LAB + 1];
   std::vector<T2Group> groups;
@@ -1483,6 +1539,8 @@ extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t
       const int key = tmpl_key(la, lb, l);
       T2Group g{t, l, ntot2, d->tmpl_cnt[key], d->tmpl_off[key], std::max(la, lb) + l};
       groups.push_back(g);
+      // per entry: 4 FLOPs per window point + 150 for the integrator
+      P.flops_t2 += (double)g.ne * (4.0 * (tr.ge - tr.gs) + 150.0);
       goff.push_back(ntot2);
       ntot2 += g.ne;
       maxNE = std::max(maxNE, g.ne);
@@ -1703,6 +1761,38 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
   if (ms_total) *ms_total = total;
   return 0;
 }
+
+// One eager evaluation with an event pair around every kernel launch on its
+// own stream; per family: launches, summed duration (ms), algorithmic FLOPs.
+extern "C" int ecp_dev_profile(ecp_dev *d, int *launches, double *ms, double *flops, char *err, size_t errlen) {
+  if (!d->prepared) {
+    snprintf(err, errlen, "ecp_dev_profile before ecp_dev_prepare");
+    return -1;
+  }
+  for (int f = 0; f < FAM_N; f++) {
+    launches[f] = 0;
+    ms[f] = 0.0;
+    flops[f] = 0.0;
+  }
+  if (d->P.nt == 0) return FAM_N;
+  d->profiling = true;
+  d->npev = 0;
+  enqueue(d, d->stream, nullptr);
+  d->profiling = false;
+  ECP_CHECK(hipGetLastError());
+  ECP_CHECK(hipStreamSynchronize(d->stream));
+  for (int i = 0; i < d->npev; i++) {
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, d->pev[2 * i], d->pev[2 * i + 1]);
+    launches[d->pfam[i]]++;
+    ms[d->pfam[i]] += t;
+  }
+  for (int c = 0; c <= ECP_MAX_LAB; c++) flops[FAM_TYPE1] += d->P.flops_t1[c];
+  flops[FAM_TYPE2] = d->P.flops_t2;
+  return FAM_N;
+}
+
+extern "C" const char *ecp_dev_family_name(int f) { return (f >= 0 && f < FAM_N) ? kFamName[f] : ""; }
 
 extern "C" int ecp_dev_fetch(ecp_dev *d, double *blocks_out, unsigned char *status_out, char *err, size_t errlen) {
   const Prepared &P = d->P;

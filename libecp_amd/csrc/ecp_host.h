@@ -84,6 +84,12 @@ int ecp_dev_run(ecp_dev *d, const ecp_job *job, double *blocks, unsigned char *s
 int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t errlen);
 int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *err, size_t errlen);
 int ecp_dev_fetch(ecp_dev *d, double *blocks, unsigned char *status, char *err, size_t errlen);
+/* One eager evaluation timed per kernel launch (HIP events on the launch's
+   own stream).  Per kernel family (tables, type1, type2, fallback1,
+   fallback2, assemble): launches, summed duration in ms, algorithmic FLOPs
+   of one evaluation (type1/type2 only).  Returns the number of families. */
+int ecp_dev_profile(ecp_dev *d, int *launches, double *ms, double *flops, char *err, size_t errlen);
+const char *ecp_dev_family_name(int f);
 /* Stage timings of the last execute (ms): tables, quadratures (small grid
    and large-grid fallbacks, concurrent), assembly[, mean graph replay]. */
 int ecp_dev_last_timing(ecp_dev *d, double *ms, int n);

@@ -156,6 +156,18 @@ class Batch:
         n = lib().libecp_batch_stage_ms(C.c_void_p(self.b), out, C.c_int(8))
         return list(out[:n])
 
+    def profile(self):
+        """{family: (launches, ms, algorithmic_flops)} for one evaluation."""
+        L = lib()
+        L.libecp_kernel_family.restype = C.c_char_p
+        la = (C.c_int * 16)()
+        ms = (C.c_double * 16)()
+        fl = (C.c_double * 16)()
+        k = L.libecp_batch_profile(C.c_void_p(self.b), la, ms, fl, C.c_int(16))
+        if k < 0:
+            raise RuntimeError("libecp_batch_profile failed: " + last_error())
+        return {L.libecp_kernel_family(i).decode(): (la[i], ms[i], fl[i]) for i in range(k)}
+
     def close(self):
         if self.b:
             lib().libecp_batch_free(C.c_void_p(self.b))
