@@ -24,6 +24,7 @@ import multiprocessing as mp
 import os
 import statistics
 import sys
+import threading
 import time
 
 import numpy as np
@@ -32,7 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from libecp_amd import native  # noqa: E402
-from libecp_amd.workloads import medium_cluster, uneven_batch  # noqa: E402
+from libecp_amd.workloads import medium_cluster, serving_requests, uneven_batch  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_medium_cluster.md")
@@ -118,6 +119,94 @@ def cpu_baseline(m, nominal):
     return (single, multi), I1
 
 
+def _pct(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def serve_bench(n_req, slots, budget_gib, max_batch, load=0.7, seed=0):
+    """Serving workload: a Poisson stream of ECP-matrix requests (5-120
+    atoms, 30% perturbed repeats) against the request server.  The arrival
+    rate is set from a measured capacity: the first requests are submitted
+    at once and their nominal triplets over the wall time to the last result
+    give the capacity; the stream then offers `load` times that."""
+    reqs = serving_requests(seed=seed, n=n_req)
+    mols = [m for m, _ in reqs]
+    nominal = np.array([m.nominal_triplets() for m in mols], dtype=np.float64)
+    S = native.Server(slots=slots, hbm_budget=int(budget_gib * (1 << 30)), max_batch_triplets=max_batch)
+    try:
+        # warm the workers (first use of each context)
+        for t in [S.submit(m) for m in mols[:slots]]:
+            S.wait(t)
+            S.result(t, matrix=False)
+        # capacity: a closed burst
+        n_cap = min(len(mols), 300)
+        t0 = time.perf_counter()
+        ts = [S.submit(m) for m in mols[:n_cap]]
+        for _ in ts:
+            t = S.next(120.0)
+            if t is None:
+                raise RuntimeError("server stalled during the capacity burst")
+            S.result(t, matrix=False)
+        cap_wall = time.perf_counter() - t0
+        capacity = nominal[:n_cap].sum() / cap_wall
+        rate = load * capacity / nominal.mean()
+        rng = np.random.default_rng(seed + 1)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(mols)))
+        tickets = [None] * len(mols)
+        lag = []
+
+        def producer():
+            start = time.perf_counter()
+            for i, m in enumerate(mols):
+                dt = arrivals[i] - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                lag.append(time.perf_counter() - start - arrivals[i])
+                tickets[i] = S.submit(m)
+
+        th = threading.Thread(target=producer)
+        th.start()
+        res = {}
+        check = {}
+        while len(res) < len(mols):
+            t = S.next(120.0)
+            if t is None:
+                raise RuntimeError("server stalled")
+            keep = t in tickets[:3] or (len(check) < 3 and t in tickets)
+            rc, I, times = S.result(t, matrix=keep)[:3]
+            res[t] = (rc, times)
+            if keep:
+                check[t] = I
+        th.join()
+        st = S.stats()
+    finally:
+        S.close()
+    idx = {t: i for i, t in enumerate(tickets)}
+    lat = [1e3 * (v[1][2] - v[1][0]) for v in res.values()]
+    wait = [1e3 * (v[1][1] - v[1][0]) for v in res.values()]
+    first = min(v[1][0] for v in res.values())
+    last = max(v[1][2] for v in res.values())
+    failed = sum(1 for v in res.values() if v[0] != 0)
+    # results in the mix against the same molecules alone (plain drop-in)
+    diff = 0.0
+    for t, I in check.items():
+        rc, I_alone = native.get_integrals(mols[idx[t]])
+        diff = max(diff, float(np.abs(I - I_alone).max()))
+    return {
+        "value": nominal.sum() / (last - first), "unit": "shell-triplets/s",
+        "p50_latency_ms": _pct(lat, 50), "p99_latency_ms": _pct(lat, 99), "mean_latency_ms": float(np.mean(lat)),
+        "p50_admission_wait_ms": _pct(wait, 50), "p99_admission_wait_ms": _pct(wait, 99),
+        "requests": len(mols), "failed": failed,
+        "offered_requests_per_s": rate, "offered_load": load,
+        "capacity_shell_triplets_per_s": capacity, "capacity_sample": f"first {n_cap} requests submitted at once",
+        "submit_lag_p99_ms": _pct([1e3 * x for x in lag], 99),
+        "slots": slots, "hbm_budget_gib": budget_gib, "max_batch_triplets": max_batch,
+        "server": st, "max_abs_mix_vs_alone": diff, "mix_vs_alone_checked": len(check),
+        "workload": f"serving_requests(seed={seed}): {len(mols)} requests, 5..120 atoms (batch recipe, l <= 3), "
+                    "30% perturbed repeats (sigma 0.05 bohr), Poisson arrivals",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +215,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gib", type=float, default=32.0)
+    ap.add_argument("--serve-max-batch", type=int, default=4000000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of 256 MiB (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -234,6 +328,14 @@ def main():
                           "stage_ms_first_step": dict(zip(["tables", "quadratures", "assembly"], st_b[:3])),
                           "workload": "uneven_batch(seed=0): 256 molecules, 3..40 atoms, 20% ECP atoms "
                                       "(L=4), basis l <= 3, one resident job"}
+        if "plain" in out:
+            out["batched"]["vs_plain"] = out["batched"]["value"] / out["plain"]["value"]
+
+    if not args.no_serve:
+        out["serving"] = serve_bench(args.serve_requests, args.serve_slots, args.serve_budget_gib,
+                                     args.serve_max_batch)
+        if "plain" in out:
+            out["serving"]["vs_plain"] = out["serving"]["value"] / out["plain"]["value"]
 
     if not args.no_cpu:
         cpu, I_ref = cpu_baseline(m, nominal)

@@ -6,6 +6,8 @@
 #ifndef LIBECP_EXT_H
 #define LIBECP_EXT_H 1
 
+#include <stddef.h>
+
 #include "libecp.h"
 
 #ifdef __cplusplus
@@ -64,6 +66,46 @@ int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n);
 int libecp_batch_profile(libecp_batch *b, int *launches, double *ms, double *flops, int n);
 const char *libecp_kernel_family(int i);
 void libecp_batch_free(libecp_batch *b);
+
+/* ---- serving ------------------------------------------------------------
+   A request server with continuous batching.  `slots` worker threads each
+   own a device context (streams, staging, resident static tables); a free
+   worker takes every queued request (up to max_batch_triplets nominal
+   triplets per job, 0 = 4,000,000), screens them on the host into one job
+   and admits the job when its device block fits in a pool of hbm_budget
+   bytes allocated once at creation.  Jobs of different workers run on the
+   device at the same time.  A request's result does not depend on what it
+   was batched with.  All calls are thread-safe. */
+
+typedef struct libecp_server libecp_server;
+
+/* Uses the calling thread's current HIP device.  NULL on error. */
+libecp_server *libecp_server_create(int slots, size_t hbm_budget, long long max_batch_triplets,
+                                    int largeGridOrder, double tolerance, double accuracy);
+/* Queues one molecule (arrays are copied; invalid input is rejected here).
+   Returns its ticket (>= 0), or -1. */
+long long libecp_server_submit(libecp_server *s, const libecp_molecule *m);
+/* Waits for a request: 0 finished, 1 timed out (timeout_s < 0: no limit),
+   -1 unknown ticket. */
+int libecp_server_wait(libecp_server *s, long long ticket, double timeout_s);
+/* The next finished request not yet returned by this call or collected, in
+   completion order; -1 if none finished within timeout_s. */
+long long libecp_server_next(libecp_server *s, double timeout_s);
+/* Collects a finished request and releases it: adds its ECP matrix into the
+   upper triangle of I (row stride rowdim; I may be NULL to discard), and
+   when times != NULL writes seconds since server creation: submitted,
+   admitted (device memory granted), finished.  Returns 0, 1 / 2 for a
+   failed type-1 / type-2 integration (blocks up to the failure added), or
+   -1 (message in libecp_last_error(): device error, a request larger than
+   the HBM budget, unknown ticket). */
+int libecp_server_result(libecp_server *s, long long ticket, int rowdim, double *I, double *times);
+/* Counters, up to n of: submitted, finished, jobs run, requests in jobs,
+   most jobs on the device at once, largest job (requests), peak pool bytes,
+   pool bytes, queued now, running now.  Returns the number available. */
+int libecp_server_stats(libecp_server *s, long long *out, int n);
+/* Stops the workers after their current jobs; unfinished requests are
+   dropped. */
+void libecp_server_free(libecp_server *s);
 
 #ifdef __cplusplus
 }

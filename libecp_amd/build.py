@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libecp_amd.so")
 
-C_SOURCES = ["ecp_tables.c", "ecp_job.c", "ecp_api.c"]
+C_SOURCES = ["ecp_tables.c", "ecp_job.c", "ecp_api.c", "ecp_serve.c"]
 HIP_SOURCES = ["ecp_device.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -52,7 +52,7 @@ def build(verbose: bool = False) -> str:
     objs = [os.path.join(BUILD, s.replace(".c", ".o")) for s in C_SOURCES] + \
            [os.path.join(BUILD, s.replace(".hip", ".o")) for s in HIP_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs, "-lm"])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs, "-lm", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

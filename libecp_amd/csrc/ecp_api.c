@@ -19,6 +19,8 @@ static void set_last_error(const char *msg) { snprintf(g_err, sizeof(g_err), "%s
 
 const char *libecp_last_error(void) { return g_err; }
 
+void ecp_api_set_error(const char *msg) { set_last_error(msg); }
+
 struct _libECPHandle {
   int nat, n;
   /* private copies of the inputs (the reference borrows them, libecp.c:68-74) */
@@ -205,31 +207,11 @@ int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *
     libECP_free(h);
     return 1;
   }
-  {
-    const ecp_job *J = &h->job;
-    /* AO offsets: atom-major, IJK(l) per shell (getIntegrals.c:59-72) */
-    int *aoff = malloc(sizeof(int) * ((size_t)J->nsh + 1));
-    int lstart = 0;
-    for (int s = 0; s < J->nsh; s++) {
-      aoff[s] = lstart;
-      lstart += ECP_DIM_IJK(J->shells[s].l);
-    }
-    for (int t = 0; t < J->ntrip; t++) {
-      const ecp_triplet *tr = &J->trip[t];
-      const int la = J->shells[tr->sa].l, lb = J->shells[tr->sb].l;
-      const int a0 = aoff[tr->sa], b0 = aoff[tr->sb];
-      const int na = ECP_DIM_IJK(la), nb = ECP_DIM_IJK(lb);
-      for (int type = 0; type < 2; type++) {
-        if (h->status[2 * t + type]) goto done; /* reference stops at the failure */
-        const double *blk = h->blocks + (type ? tr->blk2 : tr->blk1);
-        for (int i = 0; i < na; i++)
-          for (int j = 0; j < nb; j++) {
-            if (a0 + i > b0 + j) continue;
-            I[(size_t)(a0 + i) * rowdim + (b0 + j)] += blk[i * nb + j];
-          }
-      }
-    }
-  done:
+  if (h->job.ntrip > 0) {
+    int *aoff = malloc(sizeof(int) * ((size_t)h->job.nsh + 1));
+    ecp_job_ao_offsets(&h->job, aoff);
+    /* the reference stops accumulating at a failed integration */
+    (void)ecp_job_scatter(&h->job, 0, aoff, h->blocks, h->status, rowdim, 0, I);
     free(aoff);
   }
   libECP_free(h);
@@ -279,12 +261,11 @@ long long libecp_nominal_triplets(libECPHandle *h) {
 
 struct libecp_batch {
   int nmol;
-  ecp_mol *mols;
-  void **owned;
-  int nowned;
+  ecp_mol *mols; /* private copies */
   ecp_static st;
   ecp_dev *dev;
   ecp_job job;
+  int *aoff;
   double *blocks;
   unsigned char *status;
   long long nominal;
@@ -295,9 +276,9 @@ void libecp_batch_free(libecp_batch *b) {
   ecp_dev_destroy(b->dev);
   ecp_job_free(&b->job);
   ecp_static_free(&b->st);
-  for (int i = 0; i < b->nowned; i++) free(b->owned[i]);
-  free(b->owned);
+  for (int i = 0; i < b->nmol; i++) ecp_mol_release(&b->mols[i]);
   free(b->mols);
+  free(b->aoff);
   free(b->blocks);
   free(b->status);
   free(b);
@@ -314,44 +295,20 @@ libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int lar
   libecp_batch *b = calloc(1, sizeof(*b));
   b->nmol = nmol;
   b->mols = calloc((size_t)nmol, sizeof(ecp_mol));
-  b->owned = calloc((size_t)nmol * 12, sizeof(void *));
   int maxL = 1, maxLBS = 0;
   for (int i = 0; i < nmol; i++) {
     const libecp_molecule *M = &mols[i];
     ecp_mol m0 = {M->nrAtoms, M->geometry, M->shellsECP, M->lECP, M->KECP, M->nECP, M->dECP, M->aECP,
                   M->shellsBS, M->lBS, M->KBS, M->dBS, M->aBS};
     ecp_mol_dims d;
-    if (ecp_mol_scan(&m0, &d, err, sizeof(err))) {
+    if (ecp_mol_copy(&m0, &b->mols[i], &d, err, sizeof(err))) {
       char msg[600];
       snprintf(msg, sizeof(msg), "molecule %d: %s", i, err);
       set_last_error(msg);
       libecp_batch_free(b);
       return NULL;
     }
-    int necps = 0, ns = 0;
-    for (int a = 0; a < M->nrAtoms; a++) {
-      necps += M->shellsECP[a];
-      ns += M->shellsBS[a];
-      b->nominal += 0;
-    }
-    long long ne = d.necp;
-    b->nominal += ne * ns * (ns + 1) / 2;
-    void **o = b->owned + b->nowned;
-    o[0] = dupmem(M->geometry, sizeof(double) * 3 * (size_t)M->nrAtoms);
-    o[1] = dupmem(M->shellsECP, sizeof(int) * (size_t)M->nrAtoms);
-    o[2] = dupmem(M->lECP, sizeof(int) * (size_t)necps);
-    o[3] = dupmem(M->KECP, sizeof(int) * (size_t)necps);
-    o[4] = dupmem(M->nECP, sizeof(double) * (size_t)d.ngauss);
-    o[5] = dupmem(M->dECP, sizeof(double) * (size_t)d.ngauss);
-    o[6] = dupmem(M->aECP, sizeof(double) * (size_t)d.ngauss);
-    o[7] = dupmem(M->shellsBS, sizeof(int) * (size_t)M->nrAtoms);
-    o[8] = dupmem(M->lBS, sizeof(int) * (size_t)d.nsh);
-    o[9] = dupmem(M->KBS, sizeof(int) * (size_t)d.nsh);
-    o[10] = dupmem(M->dBS, sizeof(double) * (size_t)d.nprim);
-    o[11] = dupmem(M->aBS, sizeof(double) * (size_t)d.nprim);
-    b->nowned += 12;
-    ecp_mol m = {M->nrAtoms, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9], o[10], o[11]};
-    b->mols[i] = m;
+    b->nominal += (long long)d.necp * d.nsh * (d.nsh + 1) / 2;
     if (d.maxLECP > maxL) maxL = d.maxLECP;
     if (d.maxLBS > maxLBS) maxLBS = d.maxLBS;
   }
@@ -361,6 +318,8 @@ libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int lar
     libecp_batch_free(b);
     return NULL;
   }
+  b->aoff = malloc(sizeof(int) * ((size_t)b->job.nsh + 1));
+  ecp_job_ao_offsets(&b->job, b->aoff);
   b->blocks = malloc(sizeof(double) * (b->job.nblk + 1));
   b->status = calloc(2 * (size_t)b->job.ntrip + 1, 1);
   b->dev = ecp_dev_create(&b->st, err, sizeof(err));
@@ -394,40 +353,7 @@ int libecp_batch_fetch(libecp_batch *b) {
 
 int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I) {
   if (!b || mol < 0 || mol >= b->nmol) return -1;
-  const ecp_job *J = &b->job;
-  /* AO offsets of the molecule's shells */
-  int s0 = 0;
-  for (int m = 0; m < mol; m++) {
-    const ecp_mol *M = &b->mols[m];
-    for (int a = 0; a < M->nat; a++) s0 += M->shellsBS[a];
-  }
-  int nsh = 0;
-  for (int a = 0; a < b->mols[mol].nat; a++) nsh += b->mols[mol].shellsBS[a];
-  int *aoff = malloc(sizeof(int) * ((size_t)nsh + 1));
-  int lstart = 0;
-  for (int s = 0; s < nsh; s++) {
-    aoff[s] = lstart;
-    lstart += ECP_DIM_IJK(J->shells[s0 + s].l);
-  }
-  int rc = 0;
-  for (int t = J->trip_mol_begin[mol]; t < J->trip_mol_begin[mol + 1] && !rc; t++) {
-    const ecp_triplet *tr = &J->trip[t];
-    const int la = J->shells[tr->sa].l, lb = J->shells[tr->sb].l;
-    const int a0 = aoff[tr->sa - s0], b0 = aoff[tr->sb - s0];
-    const int na = ECP_DIM_IJK(la), nb = ECP_DIM_IJK(lb);
-    for (int type = 0; type < 2; type++) {
-      if (b->status[2 * t + type]) {
-        rc = type + 1;
-        break;
-      }
-      const double *blk = b->blocks + (type ? tr->blk2 : tr->blk1);
-      for (int i = 0; i < na; i++)
-        for (int j = 0; j < nb; j++)
-          if (a0 + i <= b0 + j) I[(size_t)(a0 + i) * rowdim + (b0 + j)] += blk[i * nb + j];
-    }
-  }
-  free(aoff);
-  return rc;
+  return ecp_job_scatter(&b->job, mol, b->aoff, b->blocks, b->status, rowdim, 0, I);
 }
 
 long long libecp_batch_triplets(libecp_batch *b, long long *evaluated) {

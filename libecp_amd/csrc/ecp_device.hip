@@ -1122,6 +1122,16 @@ __global__ void k_zero(unsigned int *__restrict__ p, size_t nwords) {
     p[i] = 0u;
 }
 
+// byte offsets of a job's arrays inside its device block: inputs (uploaded)
+// first, then device-only buffers
+struct JobLayout {
+  size_t i_atoms = 0, i_amaxl = 0, i_sh = 0, i_pa = 0, i_pd = 0, i_cent = 0, i_gauss = 0, i_win = 0, i_trip = 0,
+         i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, in_bytes = 0;
+  size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
+         d_q = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
+  int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0;
+};
+
 struct Prepared {
   int nt = 0;
   int ncent = 0;
@@ -1179,7 +1189,8 @@ struct ecp_dev {
   std::vector<int> tmpl, tmpl_off, tmpl_cnt;
   int *tmpl_dev = nullptr;
   Prepared P;
-  bool prepared = false;
+  JobLayout L;
+  bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   // per-launch profiling (eager runs only)
@@ -1490,17 +1501,26 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
 
 }  // namespace
 
-extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t errlen) {
+// Host half of preparing a job: class grouping of the type-1 units, type-2
+// entry groups, table work items, the algorithmic FLOP model, and the byte
+// layout of the job's device block; the inputs are staged in pinned host
+// memory.  *bytes = size of the device block the job needs.
+extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char *err, size_t errlen) {
   Prepared &P = d->P;
+  JobLayout &Ly = d->L;
   d->prepared = false;
+  d->planned = false;
+  d->eager_done = false;
   reset_graph(d);
   P = Prepared();
+  Ly = JobLayout();
   const int nt = job->ntrip;
   P.nt = nt;
   P.ncent = job->ncent;
   P.nblk = job->nblk;
+  if (bytes) *bytes = 0;
   if (nt == 0) {
-    d->prepared = true;
+    d->planned = true;
     return 0;
   }
   // type-1 units grouped by class lab, inside a class in triplet order
@@ -1530,7 +1550,7 @@ extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t
   const size_t nq = cls_qbase[ECP_MAX_LAB + 1];
   std::vector<T2Group> groups;
   std::vector<int> goff, tgfirst(nt);
-  int ntot2 = 0, maxNE = 0;
+  int ntot2 = 0;
   for (int t = 0; t < nt; t++) {
     const ecp_triplet &tr = job->trip[t];
     const int la = job->shells[tr.sa].l, lb = job->shells[tr.sb].l;
@@ -1543,7 +1563,6 @@ extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t
       P.flops_t2 += (double)g.ne * (4.0 * (tr.ge - tr.gs) + 150.0);
       goff.push_back(ntot2);
       ntot2 += g.ne;
-      maxNE = std::max(maxNE, g.ne);
       P.lmb_count[g.lmb]++;
     }
   }
@@ -1563,140 +1582,62 @@ extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t
   // layout: inputs (copied), then device-only buffers; the per-run zeroed
   // counters and flags are contiguous
   size_t in_off = 0;
-  auto in_take = [&](size_t bytes) {
+  auto in_take = [&](size_t b) {
     size_t o = in_off;
-    in_off += align_up(bytes);
+    in_off += align_up(b);
     return o;
   };
-  const size_t i_atoms = in_take(sizeof(ecp_xyz) * job->nat), i_amaxl = in_take(4 * (size_t)job->nat);
-  const size_t i_sh = in_take(sizeof(ecp_shell) * job->nsh), i_pa = in_take(8 * (size_t)job->nprim),
-               i_pd = in_take(8 * (size_t)job->nprim);
-  const size_t i_cent = in_take(sizeof(ecp_centre) * job->ncent),
-               i_gauss = in_take(sizeof(ecp_gauss) * job->ngauss);
-  const size_t i_win = in_take(sizeof(ecp_window) * job->nwin), i_trip = in_take(sizeof(ecp_triplet) * nt);
-  const size_t i_u1 = in_take(sizeof(T1Unit) * units1.size()), i_t1f = in_take(4 * (size_t)nt);
-  const size_t i_grp = in_take(sizeof(T2Group) * groups.size()), i_goff = in_take(4 * goff.size());
-  const size_t i_tgf = in_take(4 * (size_t)nt), i_fit = in_take(sizeof(int2) * fitems.size()),
-               i_ait = in_take(sizeof(int2) * aitems.size());
-  const size_t in_bytes = in_off;
-  size_t dv_off = align_up(in_bytes);
-  auto dv_take = [&](size_t bytes) {
+  Ly.i_atoms = in_take(sizeof(ecp_xyz) * job->nat);
+  Ly.i_amaxl = in_take(4 * (size_t)job->nat);
+  Ly.i_sh = in_take(sizeof(ecp_shell) * job->nsh);
+  Ly.i_pa = in_take(8 * (size_t)job->nprim);
+  Ly.i_pd = in_take(8 * (size_t)job->nprim);
+  Ly.i_cent = in_take(sizeof(ecp_centre) * job->ncent);
+  Ly.i_gauss = in_take(sizeof(ecp_gauss) * job->ngauss);
+  Ly.i_win = in_take(sizeof(ecp_window) * job->nwin);
+  Ly.i_trip = in_take(sizeof(ecp_triplet) * nt);
+  Ly.i_u1 = in_take(sizeof(T1Unit) * units1.size());
+  Ly.i_t1f = in_take(4 * (size_t)nt);
+  Ly.i_grp = in_take(sizeof(T2Group) * groups.size());
+  Ly.i_goff = in_take(4 * goff.size());
+  Ly.i_tgf = in_take(4 * (size_t)nt);
+  Ly.i_fit = in_take(sizeof(int2) * fitems.size());
+  Ly.i_ait = in_take(sizeof(int2) * aitems.size());
+  Ly.in_bytes = in_off;
+  size_t dv_off = align_up(Ly.in_bytes);
+  auto dv_take = [&](size_t b) {
     size_t o = dv_off;
-    dv_off += align_up(bytes);
+    dv_off += align_up(b);
     return o;
   };
-  const size_t d_zero = dv_off;
-  const size_t d_fbc = dv_take(4 * 64), d_gflag = dv_take(4 * (size_t)P.ngroups), d_stat = dv_take(2 * (size_t)nt);
-  const size_t d_zero_end = dv_off;
-  const size_t d_UL = dv_take(8 * job->nUL), d_Ut = dv_take(8 * job->nUt), d_F = dv_take(8 * job->nF),
-               d_ang = dv_take(8 * job->nang);
-  const size_t d_q = dv_take(8 * nq), d_qf = dv_take(8 * units1.size());
-  const size_t d_fbq1 = dv_take(4 * units1.size());
-  const size_t d_T = dv_take(8 * (size_t)ntot2), d_tf = dv_take((size_t)ntot2);
-  const size_t d_fbq2 = dv_take(4 * (size_t)(kMaxLMB + 1) * P.ngroups);
-  const size_t d_fbT = dv_take(8 * (size_t)ntot2 * P.maxPairs);
-  const size_t d_blk = dv_take(8 * job->nblk);
-  const size_t total = dv_off;
-  if (total > d->arena_cap) {
-    if (d->arena) ECP_CHECK(hipFree(d->arena));
-    d->arena = nullptr;
-    d->arena_cap = 0;
-    const size_t cap = total + total / 4;
-    ECP_CHECK(hipMalloc(&d->arena, cap));
-    d->arena_cap = cap;
-  }
-  if (in_bytes > d->pin_in_cap) {
-    if (d->pin_in) ECP_CHECK(hipHostFree(d->pin_in));
-    d->pin_in = nullptr;
-    ECP_CHECK(hipHostMalloc(&d->pin_in, in_bytes + in_bytes / 4));
-    d->pin_in_cap = in_bytes + in_bytes / 4;
-  }
-  const size_t out_bytes = 8 * job->nblk + 2 * (size_t)nt;
-  if (out_bytes > d->pin_out_cap) {
-    if (d->pin_out) ECP_CHECK(hipHostFree(d->pin_out));
-    d->pin_out = nullptr;
-    ECP_CHECK(hipHostMalloc(&d->pin_out, out_bytes + out_bytes / 4));
-    d->pin_out_cap = out_bytes + out_bytes / 4;
-  }
-  char *H = d->pin_in;
-  auto cp = [&](size_t o, const void *src, size_t bytes) {
-    if (bytes) memcpy(H + o, src, bytes);
-  };
-  cp(i_atoms, job->atoms, sizeof(ecp_xyz) * job->nat);
-  cp(i_amaxl, job->atom_maxl, 4 * (size_t)job->nat);
-  cp(i_sh, job->shells, sizeof(ecp_shell) * job->nsh);
-  cp(i_pa, job->pa, 8 * (size_t)job->nprim);
-  cp(i_pd, job->pd, 8 * (size_t)job->nprim);
-  cp(i_cent, job->cent, sizeof(ecp_centre) * job->ncent);
-  cp(i_gauss, job->gauss, sizeof(ecp_gauss) * job->ngauss);
-  cp(i_win, job->win, sizeof(ecp_window) * job->nwin);
-  cp(i_trip, job->trip, sizeof(ecp_triplet) * nt);
-  cp(i_u1, units1.data(), sizeof(T1Unit) * units1.size());
-  cp(i_t1f, t1first.data(), 4 * (size_t)nt);
-  cp(i_grp, groups.data(), sizeof(T2Group) * groups.size());
-  cp(i_goff, goff.data(), 4 * goff.size());
-  cp(i_tgf, tgfirst.data(), 4 * (size_t)nt);
-  cp(i_fit, fitems.data(), sizeof(int2) * fitems.size());
-  cp(i_ait, aitems.data(), sizeof(int2) * aitems.size());
-  char *A = d->arena;
-  ECP_CHECK(hipMemcpyAsync(A, H, in_bytes, hipMemcpyHostToDevice, d->stream));
-  ECP_CHECK(hipStreamSynchronize(d->stream));
-  DevJob &J = P.J;
-  J.atoms = (const ecp_xyz *)(A + i_atoms);
-  J.atom_maxl = (const int *)(A + i_amaxl);
-  J.shells = (const ecp_shell *)(A + i_sh);
-  J.pa = (const double *)(A + i_pa);
-  J.pd = (const double *)(A + i_pd);
-  J.cent = (const ecp_centre *)(A + i_cent);
-  J.gauss = (const ecp_gauss *)(A + i_gauss);
-  J.win = (const ecp_window *)(A + i_win);
-  J.trip = (const ecp_triplet *)(A + i_trip);
-  J.ntrip = nt;
-  J.UL = (double *)(A + d_UL);
-  J.Ut = (double *)(A + d_Ut);
-  J.F = (double *)(A + d_F);
-  J.ang = (double *)(A + d_ang);
-  J.FL = job->FL;
-  J.UK = job->UK;
-  J.angL = job->angL;
-  J.ang_lam = job->ang_lam;
-  J.ang_c = job->ang_c;
-  J.maxPairs = P.maxPairs;
-  P.u1 = (const T1Unit *)(A + i_u1);
-  P.q = (double *)(A + d_q);
-  P.qf = (unsigned long long *)(A + d_qf);
-  P.fbq1 = (int *)(A + d_fbq1);
-  P.fbc = (int *)(A + d_fbc);  // [0..10] type-1 queues, [16..27] type-2 queues
-  P.stat = (unsigned char *)(A + d_stat);
-  P.grp = (const T2Group *)(A + i_grp);
-  P.goff = (const int *)(A + i_goff);
-  P.Tout = (double *)(A + d_T);
-  P.tf = (unsigned char *)(A + d_tf);
-  P.gflag = (int *)(A + d_gflag);
-  P.fbq2 = (int *)(A + d_fbq2);
-  P.fbT = (double *)(A + d_fbT);
-  P.fit = (const int2 *)(A + i_fit);
-  P.ait = (const int2 *)(A + i_ait);
-  P.blk = (double *)(A + d_blk);
-  P.zero_base = (unsigned int *)(A + d_zero);
-  P.zero_words = (d_zero_end - d_zero) / 4;
-  AsmArgs &aa = P.aa;
-  aa.units1 = P.u1;
-  aa.t1first = (const int *)(A + i_t1f);
-  aa.qout = P.q;
-  aa.groups = P.grp;
-  aa.tgfirst = (const int *)(A + i_tgf);
-  aa.tmpl = d->tmpl_dev;
-  aa.Tout = P.Tout;
-  aa.tfail = P.tf;
-  aa.fbT = P.fbT;
-  aa.fac = d->fac_dev;
-  aa.dfac = d->dfac_dev;
-  aa.blocks = P.blk;
+  Ly.d_zero = dv_off;
+  Ly.d_fbc = dv_take(4 * 64);
+  Ly.d_gflag = dv_take(4 * (size_t)P.ngroups);
+  Ly.d_stat = dv_take(2 * (size_t)nt);
+  Ly.d_zero_end = dv_off;
+  Ly.d_UL = dv_take(8 * job->nUL);
+  Ly.d_Ut = dv_take(8 * job->nUt);
+  Ly.d_F = dv_take(8 * job->nF);
+  Ly.d_ang = dv_take(8 * job->nang);
+  Ly.d_q = dv_take(8 * nq);
+  Ly.d_qf = dv_take(8 * units1.size());
+  Ly.d_fbq1 = dv_take(4 * units1.size());
+  Ly.d_T = dv_take(8 * (size_t)ntot2);
+  Ly.d_tf = dv_take((size_t)ntot2);
+  Ly.d_fbq2 = dv_take(4 * (size_t)(kMaxLMB + 1) * P.ngroups);
+  Ly.d_fbT = dv_take(8 * (size_t)ntot2 * P.maxPairs);
+  Ly.d_blk = dv_take(8 * job->nblk);
+  Ly.total = dv_off;
+  Ly.FL = job->FL;
+  Ly.UK = job->UK;
+  Ly.angL = job->angL;
+  Ly.ang_lam = job->ang_lam;
+  Ly.ang_c = job->ang_c;
   {
     int LM = 0, LC = 1;
     for (int i = 0; i < job->nsh; i++) LM = std::max(LM, job->shells[i].l);
     for (int c = 0; c < job->ncent; c++) LC = std::max(LC, job->cent[c].L);
+    AsmArgs &aa = P.aa;
     aa.pair_chunk = (LM <= 4) ? 16 : 8;
     aa.lds_rs_stride = (2 * LM + 1) * (2 * LM + 1);
     int o = 0;
@@ -1711,12 +1652,144 @@ extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t
       return -1;
     }
   }
+  if (Ly.in_bytes > d->pin_in_cap) {
+    if (d->pin_in) ECP_CHECK(hipHostFree(d->pin_in));
+    d->pin_in = nullptr;
+    d->pin_in_cap = 0;
+    ECP_CHECK(hipHostMalloc(&d->pin_in, Ly.in_bytes + Ly.in_bytes / 4));
+    d->pin_in_cap = Ly.in_bytes + Ly.in_bytes / 4;
+  }
+  const size_t out_bytes = 8 * job->nblk + 2 * (size_t)nt;
+  if (out_bytes > d->pin_out_cap) {
+    if (d->pin_out) ECP_CHECK(hipHostFree(d->pin_out));
+    d->pin_out = nullptr;
+    d->pin_out_cap = 0;
+    ECP_CHECK(hipHostMalloc(&d->pin_out, out_bytes + out_bytes / 4));
+    d->pin_out_cap = out_bytes + out_bytes / 4;
+  }
+  char *H = d->pin_in;
+  auto cp = [&](size_t o, const void *src, size_t b) {
+    if (b) memcpy(H + o, src, b);
+  };
+  cp(Ly.i_atoms, job->atoms, sizeof(ecp_xyz) * job->nat);
+  cp(Ly.i_amaxl, job->atom_maxl, 4 * (size_t)job->nat);
+  cp(Ly.i_sh, job->shells, sizeof(ecp_shell) * job->nsh);
+  cp(Ly.i_pa, job->pa, 8 * (size_t)job->nprim);
+  cp(Ly.i_pd, job->pd, 8 * (size_t)job->nprim);
+  cp(Ly.i_cent, job->cent, sizeof(ecp_centre) * job->ncent);
+  cp(Ly.i_gauss, job->gauss, sizeof(ecp_gauss) * job->ngauss);
+  cp(Ly.i_win, job->win, sizeof(ecp_window) * job->nwin);
+  cp(Ly.i_trip, job->trip, sizeof(ecp_triplet) * nt);
+  cp(Ly.i_u1, units1.data(), sizeof(T1Unit) * units1.size());
+  cp(Ly.i_t1f, t1first.data(), 4 * (size_t)nt);
+  cp(Ly.i_grp, groups.data(), sizeof(T2Group) * groups.size());
+  cp(Ly.i_goff, goff.data(), 4 * goff.size());
+  cp(Ly.i_tgf, tgfirst.data(), 4 * (size_t)nt);
+  cp(Ly.i_fit, fitems.data(), sizeof(int2) * fitems.size());
+  cp(Ly.i_ait, aitems.data(), sizeof(int2) * aitems.size());
+  if (bytes) *bytes = Ly.total;
+  d->planned = true;
+  return 0;
+}
+
+// Device half: places the planned job at `mem` (capacity cap; NULL = the
+// context's own grow-only arena), uploads the staged inputs and waits.
+extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t errlen) {
+  if (!d->planned) {
+    snprintf(err, errlen, "ecp_dev_bind before ecp_dev_plan");
+    return -1;
+  }
+  Prepared &P = d->P;
+  const JobLayout &Ly = d->L;
+  d->prepared = false;
+  d->eager_done = false;
+  reset_graph(d);
+  if (P.nt == 0) {
+    d->prepared = true;
+    return 0;
+  }
+  char *A = (char *)mem;
+  if (!A) {
+    if (Ly.total > d->arena_cap) {
+      if (d->arena) ECP_CHECK(hipFree(d->arena));
+      d->arena = nullptr;
+      d->arena_cap = 0;
+      const size_t c2 = Ly.total + Ly.total / 4;
+      ECP_CHECK(hipMalloc(&d->arena, c2));
+      d->arena_cap = c2;
+    }
+    A = d->arena;
+  } else if (Ly.total > cap) {
+    snprintf(err, errlen, "job needs %zu bytes of device memory, %zu given", Ly.total, cap);
+    return -1;
+  }
+  ECP_CHECK(hipMemcpyAsync(A, d->pin_in, Ly.in_bytes, hipMemcpyHostToDevice, d->stream));
+  ECP_CHECK(hipStreamSynchronize(d->stream));
+  DevJob &J = P.J;
+  J.atoms = (const ecp_xyz *)(A + Ly.i_atoms);
+  J.atom_maxl = (const int *)(A + Ly.i_amaxl);
+  J.shells = (const ecp_shell *)(A + Ly.i_sh);
+  J.pa = (const double *)(A + Ly.i_pa);
+  J.pd = (const double *)(A + Ly.i_pd);
+  J.cent = (const ecp_centre *)(A + Ly.i_cent);
+  J.gauss = (const ecp_gauss *)(A + Ly.i_gauss);
+  J.win = (const ecp_window *)(A + Ly.i_win);
+  J.trip = (const ecp_triplet *)(A + Ly.i_trip);
+  J.ntrip = P.nt;
+  J.UL = (double *)(A + Ly.d_UL);
+  J.Ut = (double *)(A + Ly.d_Ut);
+  J.F = (double *)(A + Ly.d_F);
+  J.ang = (double *)(A + Ly.d_ang);
+  J.FL = Ly.FL;
+  J.UK = Ly.UK;
+  J.angL = Ly.angL;
+  J.ang_lam = Ly.ang_lam;
+  J.ang_c = Ly.ang_c;
+  J.maxPairs = P.maxPairs;
+  P.u1 = (const T1Unit *)(A + Ly.i_u1);
+  P.q = (double *)(A + Ly.d_q);
+  P.qf = (unsigned long long *)(A + Ly.d_qf);
+  P.fbq1 = (int *)(A + Ly.d_fbq1);
+  P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 queues, [16..27] type-2 queues
+  P.stat = (unsigned char *)(A + Ly.d_stat);
+  P.grp = (const T2Group *)(A + Ly.i_grp);
+  P.goff = (const int *)(A + Ly.i_goff);
+  P.Tout = (double *)(A + Ly.d_T);
+  P.tf = (unsigned char *)(A + Ly.d_tf);
+  P.gflag = (int *)(A + Ly.d_gflag);
+  P.fbq2 = (int *)(A + Ly.d_fbq2);
+  P.fbT = (double *)(A + Ly.d_fbT);
+  P.fit = (const int2 *)(A + Ly.i_fit);
+  P.ait = (const int2 *)(A + Ly.i_ait);
+  P.blk = (double *)(A + Ly.d_blk);
+  P.zero_base = (unsigned int *)(A + Ly.d_zero);
+  P.zero_words = (Ly.d_zero_end - Ly.d_zero) / 4;
+  AsmArgs &aa = P.aa;
+  aa.units1 = P.u1;
+  aa.t1first = (const int *)(A + Ly.i_t1f);
+  aa.qout = P.q;
+  aa.groups = P.grp;
+  aa.tgfirst = (const int *)(A + Ly.i_tgf);
+  aa.tmpl = d->tmpl_dev;
+  aa.Tout = P.Tout;
+  aa.tfail = P.tf;
+  aa.fbT = P.fbT;
+  aa.fac = d->fac_dev;
+  aa.dfac = d->dfac_dev;
+  aa.blocks = P.blk;
   d->prepared = true;
   return 0;
 }
 
-// Runs the prepared evaluation `repeats` times (the first run records the
-// stage timings; later runs replay a captured HIP graph) and waits.
+extern "C" int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t errlen) {
+  if (ecp_dev_plan(d, job, nullptr, err, errlen)) return -1;
+  return ecp_dev_bind(d, nullptr, 0, err, errlen);
+}
+
+// Runs the prepared evaluation `repeats` times and waits.  The first
+// evaluation of a job runs eagerly and records the stage timings; when more
+// evaluations follow (in this call or a later one) the kernel sequence is
+// captured once into a HIP graph and replayed.
 extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *err, size_t errlen) {
   if (!d->prepared) {
     snprintf(err, errlen, "ecp_dev_execute before ecp_dev_prepare");
@@ -1728,9 +1801,7 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
   hipStream_t s = d->stream;
   int remaining = repeats;
   double total = 0.0;
-  if (!d->gexec) {
-    // first evaluation of a prepared job: eager, with stage timings, then
-    // capture the same sequence for replays
+  if (!d->eager_done) {
     enqueue(d, s, d->ev);
     ECP_CHECK(hipGetLastError());
     ECP_CHECK(hipStreamSynchronize(s));
@@ -1741,11 +1812,14 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
     }
     d->nev = 3;
     total = d->last_ms[0] + d->last_ms[1] + d->last_ms[2];
+    d->eager_done = true;
+    remaining--;
+  }
+  if (remaining > 0 && !d->gexec) {
     ECP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     enqueue(d, s, nullptr);
     ECP_CHECK(hipStreamEndCapture(s, &d->graph));
     ECP_CHECK(hipGraphInstantiate(&d->gexec, d->graph, nullptr, nullptr, 0));
-    remaining--;
   }
   if (remaining > 0) {
     ECP_CHECK(hipEventRecord(d->ev[5], s));
@@ -1811,3 +1885,20 @@ extern "C" int ecp_dev_run(ecp_dev *d, const ecp_job *job, double *blocks_out, u
   if (ecp_dev_execute(d, 1, nullptr, err, errlen)) return -1;
   return ecp_dev_fetch(d, blocks_out, status_out, err, errlen);
 }
+
+extern "C" int ecp_dev_mem_alloc(void **p, size_t bytes, char *err, size_t errlen) {
+  *p = nullptr;
+  ECP_CHECK(hipMalloc(p, bytes));
+  return 0;
+}
+
+extern "C" void ecp_dev_mem_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+extern "C" int ecp_dev_current(void) {
+  int dv = 0;
+  return hipGetDevice(&dv) == hipSuccess ? dv : -1;
+}
+
+extern "C" int ecp_dev_use(int device) { return hipSetDevice(device) == hipSuccess ? 0 : -1; }

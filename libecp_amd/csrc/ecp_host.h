@@ -82,6 +82,12 @@ int ecp_dev_run(ecp_dev *d, const ecp_job *job, double *blocks, unsigned char *s
    execute its kernel sequence `repeats` times (graph replays after the
    first; *ms_total = device time per repeat), and copy the results back. */
 int ecp_dev_prepare(ecp_dev *d, const ecp_job *job, char *err, size_t errlen);
+/* prepare = plan + bind.  plan does the host work (work lists, layout,
+   staging in pinned memory) and returns the bytes of device memory the job
+   needs; bind places the job at mem (capacity cap; NULL = the context's own
+   arena) and uploads its inputs. */
+int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char *err, size_t errlen);
+int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t errlen);
 int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *err, size_t errlen);
 int ecp_dev_fetch(ecp_dev *d, double *blocks, unsigned char *status, char *err, size_t errlen);
 /* One eager evaluation timed per kernel launch (HIP events on the launch's
@@ -93,6 +99,11 @@ const char *ecp_dev_family_name(int f);
 /* Stage timings of the last execute (ms): tables, quadratures (small grid
    and large-grid fallbacks, concurrent), assembly[, mean graph replay]. */
 int ecp_dev_last_timing(ecp_dev *d, double *ms, int n);
+/* device memory and the calling thread's current HIP device */
+int ecp_dev_mem_alloc(void **p, size_t bytes, char *err, size_t errlen);
+void ecp_dev_mem_free(void *p);
+int ecp_dev_current(void);
+int ecp_dev_use(int device);
 
 /* ---- molecules and jobs (ecp_job.c) ---- */
 typedef struct {
@@ -115,6 +126,11 @@ void ecp_static_free(ecp_static *st);
 int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *job, char *err,
                   size_t errlen);
 void ecp_job_free(ecp_job *job);
+int ecp_mol_copy(const ecp_mol *src, ecp_mol *dst, ecp_mol_dims *d, char *err, size_t errlen);
+void ecp_mol_release(ecp_mol *m);
+void ecp_job_ao_offsets(const ecp_job *J, int *aoff);
+int ecp_job_scatter(const ecp_job *J, int m, const int *aoff, const double *blocks, const unsigned char *status,
+                    int rowdim, int packed_n, double *I);
 
 /* ---- host helpers (ecp_job.c) ---- */
 double ecp_eval_ecp_host(const ecp_gauss *g, int ng, int l, double r);

@@ -439,3 +439,94 @@ int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *
   if (rc) ecp_job_free(job);
   return rc;
 }
+
+/* Deep copy of a molecule's input arrays into one allocation (released by
+   ecp_mol_release); validates it first.  Fills *d with its dimensions. */
+int ecp_mol_copy(const ecp_mol *src, ecp_mol *dst, ecp_mol_dims *d, char *err, size_t errlen) {
+  memset(dst, 0, sizeof(*dst));
+  if (ecp_mol_scan(src, d, err, errlen)) return -1;
+  int necps = 0;
+  for (int i = 0; i < src->nat; i++) necps += src->shellsECP[i];
+  const size_t nd = 3 * (size_t)src->nat + 3 * (size_t)d->ngauss + 2 * (size_t)d->nprim;
+  const size_t ni = 2 * (size_t)src->nat + 2 * (size_t)necps + 2 * (size_t)d->nsh;
+  double *fb = malloc(sizeof(double) * nd + sizeof(int) * ni + 8);
+  if (!fb) {
+    set_err(err, errlen, "out of host memory");
+    return -1;
+  }
+  double *f = fb;
+  int *ib = (int *)(fb + nd), *ip = ib;
+#define TAKE_D(dstp, s, n)                        \
+  do {                                            \
+    memcpy(f, (s), sizeof(double) * (size_t)(n)); \
+    (dstp) = f;                                   \
+    f += (n);                                     \
+  } while (0)
+#define TAKE_I(dstp, s, n)                         \
+  do {                                             \
+    memcpy(ip, (s), sizeof(int) * (size_t)(n));    \
+    (dstp) = ip;                                   \
+    ip += (n);                                     \
+  } while (0)
+  dst->nat = src->nat;
+  TAKE_D(dst->geom, src->geom, 3 * (size_t)src->nat);
+  TAKE_D(dst->nECP, src->nECP, d->ngauss);
+  TAKE_D(dst->dECP, src->dECP, d->ngauss);
+  TAKE_D(dst->aECP, src->aECP, d->ngauss);
+  TAKE_D(dst->dBS, src->dBS, d->nprim);
+  TAKE_D(dst->aBS, src->aBS, d->nprim);
+  TAKE_I(dst->shellsECP, src->shellsECP, src->nat);
+  TAKE_I(dst->lECP, src->lECP, necps);
+  TAKE_I(dst->KECP, src->KECP, necps);
+  TAKE_I(dst->shellsBS, src->shellsBS, src->nat);
+  TAKE_I(dst->lBS, src->lBS, d->nsh);
+  TAKE_I(dst->KBS, src->KBS, d->nsh);
+#undef TAKE_D
+#undef TAKE_I
+  return 0;
+}
+
+void ecp_mol_release(ecp_mol *m) {
+  free((void *)m->geom);
+  memset(m, 0, sizeof(*m));
+}
+
+/* AO offset of every shell of a job inside its own molecule's matrix:
+   atom-major, IJK(l) functions per shell (getIntegrals.c:59-72). */
+void ecp_job_ao_offsets(const ecp_job *J, int *aoff) {
+  int cur = -1, off = 0;
+  for (int s = 0; s < J->nsh; s++) {
+    if (J->shells[s].mol != cur) {
+      cur = J->shells[s].mol;
+      off = 0;
+    }
+    aoff[s] = off;
+    off += ECP_DIM_IJK(J->shells[s].l);
+  }
+}
+
+/* Adds molecule m's blocks into the upper triangle (row <= column) of its
+   AO matrix, as getIntegrals' callback does (getIntegrals.c:22-43): dense
+   with row stride rowdim when packed_n == 0, else packed by rows (row i
+   holds columns i..packed_n-1).  Stops at the first failed integration in
+   callback order and returns its code (1 type 1, 2 type 2), else 0. */
+int ecp_job_scatter(const ecp_job *J, int m, const int *aoff, const double *blocks, const unsigned char *status,
+                    int rowdim, int packed_n, double *I) {
+  for (int t = J->trip_mol_begin[m]; t < J->trip_mol_begin[m + 1]; t++) {
+    const ecp_triplet *tr = &J->trip[t];
+    const int la = J->shells[tr->sa].l, lb = J->shells[tr->sb].l;
+    const int a0 = aoff[tr->sa], b0 = aoff[tr->sb];
+    const int na = ECP_DIM_IJK(la), nb = ECP_DIM_IJK(lb);
+    for (int type = 0; type < 2; type++) {
+      if (status[2 * t + type]) return type + 1;
+      const double *blk = blocks + (type ? tr->blk2 : tr->blk1);
+      for (int i = 0; i < na; i++) {
+        const size_t r = (size_t)(a0 + i);
+        double *row = packed_n ? I + r * packed_n - r * (r - 1) / 2 - r : I + r * rowdim;
+        for (int j = 0; j < nb; j++)
+          if (a0 + i <= b0 + j) row[b0 + j] += blk[i * nb + j];
+      }
+    }
+  }
+  return 0;
+}

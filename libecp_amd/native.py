@@ -178,3 +178,70 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """libecp_server_*: requests evaluated with continuous batching on a
+    shared device-memory budget.  ``submit`` returns a ticket; ``next``
+    returns tickets in completion order; ``result`` collects one."""
+
+    STATS = ("submitted", "finished", "jobs", "requests_in_jobs", "max_jobs_on_device", "max_job_requests",
+             "pool_peak_bytes", "pool_bytes", "queued", "running")
+
+    def __init__(self, slots=4, hbm_budget=8 << 30, max_batch_triplets=0, grid=1024, tol=1e-12, acc=1e-14):
+        L = lib()
+        L.libecp_server_create.restype = C.c_void_p
+        L.libecp_server_create.argtypes = [C.c_int, C.c_size_t, C.c_longlong, C.c_int, C.c_double, C.c_double]
+        L.libecp_server_submit.restype = C.c_longlong
+        L.libecp_server_submit.argtypes = [C.c_void_p, C.POINTER(_Mol)]
+        L.libecp_server_wait.argtypes = [C.c_void_p, C.c_longlong, C.c_double]
+        L.libecp_server_next.restype = C.c_longlong
+        L.libecp_server_next.argtypes = [C.c_void_p, C.c_double]
+        L.libecp_server_result.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p]
+        L.libecp_server_stats.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.libecp_server_free.argtypes = [C.c_void_p]
+        self.s = L.libecp_server_create(slots, hbm_budget, max_batch_triplets, grid, tol, acc)
+        if not self.s:
+            raise RuntimeError("libecp_server_create failed: " + last_error())
+        self.nao = {}
+
+    def submit(self, m):
+        st = _mol_struct(m)
+        t = lib().libecp_server_submit(self.s, C.byref(st))
+        if t < 0:
+            raise RuntimeError("libecp_server_submit failed: " + last_error())
+        self.nao[t] = m.nao
+        return t
+
+    def wait(self, ticket, timeout=-1.0):
+        return lib().libecp_server_wait(self.s, ticket, timeout)
+
+    def next(self, timeout=-1.0):
+        t = lib().libecp_server_next(self.s, timeout)
+        return None if t < 0 else t
+
+    def result(self, ticket, matrix=True):
+        """(rc, matrix or None, (t_submit, t_admit, t_done)); rc < 0 carries
+        the error text as a fourth element."""
+        nao = self.nao.pop(ticket)
+        I = np.zeros((nao, nao)) if matrix else None
+        times = (C.c_double * 3)()
+        rc = lib().libecp_server_result(self.s, ticket, nao, None if I is None else I.ctypes.data, times)
+        out = (rc, I, tuple(times))
+        return out + (last_error(),) if rc < 0 else out
+
+    def stats(self):
+        v = (C.c_longlong * 16)()
+        k = lib().libecp_server_stats(self.s, v, 16)
+        return dict(zip(self.STATS, v[:k]))
+
+    def close(self):
+        if self.s:
+            lib().libecp_server_free(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

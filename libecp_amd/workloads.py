@@ -212,6 +212,27 @@ def perturb(rng, mol, sigma=0.05):
     return m
 
 
+def serving_requests(seed=0, n=2000, nat_range=(5, 120), frac_perturbed=0.3, sigma=0.05):
+    """Request stream of the serving workload: molecules of the batch recipe
+    with atom counts uniform in [5, 120]; a fraction ``frac_perturbed`` of the
+    requests are an earlier request's geometry displaced by N(0, sigma^2)
+    bohr per coordinate.  Returns [(molecule, index of the perturbed
+    original or -1)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        if out and rng.random() < frac_perturbed:
+            j = int(rng.integers(len(out)))
+            m = perturb(rng, out[j][0], sigma)
+            m.name = f"serving(seed={seed})[{i}] = perturbed [{j}]"
+            out.append((m, j))
+        else:
+            m = batch_molecule(rng, int(rng.integers(nat_range[0], nat_range[1] + 1)))
+            m.name = f"serving(seed={seed})[{i}]"
+            out.append((m, -1))
+    return out
+
+
 CONFIGS = {
     "plain_small": plain_small,
     "medium_cluster": medium_cluster,
