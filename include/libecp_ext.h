@@ -65,6 +65,18 @@ int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n);
    model is defined).  Returns the number of families, or -1. */
 int libecp_batch_profile(libecp_batch *b, int *launches, double *ms, double *flops, int n);
 const char *libecp_kernel_family(int i);
+/* Launch parameters of the batch's device pipeline, by name:
+   "t1_head_coop" / "t1_head_serial" (PS93 levels the type-1 head kernel runs
+   before the wave-cooperative / lane-serial tail), "t1_coop_max" (largest
+   type-1 class, in primitive pairs, that takes the cooperative tail),
+   "t1_tail_grid" (most workgroups of a tail launch).  Results do not depend
+   on them.  0, or -1 for an unknown name. */
+int libecp_batch_set_param(libecp_batch *b, const char *name, int value);
+int libecp_batch_get_param(libecp_batch *b, const char *name, int *value);
+/* Autotuner: times the batch (graph replays, `repeats` per candidate) over
+   the launch parameters by coordinate search, keeps the fastest setting,
+   and writes what it tried to report (len bytes).  0, or -1. */
+int libecp_batch_autotune(libecp_batch *b, int repeats, char *report, int len);
 void libecp_batch_free(libecp_batch *b);
 
 /* ---- serving ------------------------------------------------------------

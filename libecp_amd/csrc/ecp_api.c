@@ -386,3 +386,105 @@ int libecp_batch_profile(libecp_batch *b, int *launches, double *ms, double *flo
 }
 
 const char *libecp_kernel_family(int i) { return ecp_dev_family_name(i); }
+
+/* ---------------------------------------------------------------------- */
+/* Launch parameters and autotuning of a batch (include/libecp_ext.h).     */
+
+static int *tuning_field(ecp_tuning *t, const char *name) {
+  if (!strcmp(name, "t1_head_coop")) return &t->t1_head_coop;
+  if (!strcmp(name, "t1_head_serial")) return &t->t1_head_serial;
+  if (!strcmp(name, "t1_coop_max")) return &t->t1_coop_max;
+  if (!strcmp(name, "t1_tail_grid")) return &t->t1_tail_grid;
+  return NULL;
+}
+
+int libecp_batch_set_param(libecp_batch *b, const char *name, int value) {
+  ecp_tuning t;
+  if (!b || !name) return -1;
+  ecp_dev_get_tuning(b->dev, &t);
+  int *f = tuning_field(&t, name);
+  if (!f) {
+    set_last_error("unknown launch parameter");
+    return -1;
+  }
+  *f = value;
+  ecp_dev_set_tuning(b->dev, &t);
+  return 0;
+}
+
+int libecp_batch_get_param(libecp_batch *b, const char *name, int *value) {
+  ecp_tuning t;
+  if (!b || !name || !value) return -1;
+  ecp_dev_get_tuning(b->dev, &t);
+  int *f = tuning_field(&t, name);
+  if (!f) {
+    set_last_error("unknown launch parameter");
+    return -1;
+  }
+  *value = *f;
+  return 0;
+}
+
+/* device time (ms) of one evaluation under tuning t: graph replays */
+static double time_tuning(libecp_batch *b, const ecp_tuning *t, int repeats, char *err, size_t errlen) {
+  double ms = 0.0;
+  ecp_dev_set_tuning(b->dev, t);
+  if (ecp_dev_execute(b->dev, 1 + repeats, &ms, err, errlen)) return -1.0;
+  return ms;
+}
+
+/* Coordinate search over the launch parameters, one at a time, keeping a
+   value only when it is faster than the best so far by more than 1%. */
+int libecp_batch_autotune(libecp_batch *b, int repeats, char *report, int len) {
+  char err[512] = "";
+  if (!b) return -1;
+  if (repeats < 1) repeats = 3;
+  ecp_tuning best;
+  ecp_dev_get_tuning(b->dev, &best);
+  double tbest = time_tuning(b, &best, repeats, err, sizeof(err));
+  if (tbest < 0) {
+    set_last_error(err);
+    return -1;
+  }
+  size_t used = 0;
+  if (report && len > 0) {
+    report[0] = 0;
+    used = (size_t)snprintf(report, (size_t)len, "start %.4f ms", tbest);
+  }
+  static const struct {
+    const char *name;
+    int n;
+    int v[6];
+  } space[] = {
+      {"t1_coop_max", 5, {0, 20000, 100000, 400000, 1 << 30}},
+      {"t1_head_coop", 4, {6, 8, 10, 12}},
+      {"t1_head_serial", 4, {6, 8, 10, 15}},
+      {"t1_tail_grid", 4, {2048, 8192, 32768, 131072}},
+  };
+  for (size_t k = 0; k < sizeof(space) / sizeof(space[0]); k++) {
+    for (int i = 0; i < space[k].n; i++) {
+      ecp_tuning t = best;
+      int *f = tuning_field(&t, space[k].name);
+      if (*f == space[k].v[i]) continue;
+      *f = space[k].v[i];
+      const double ms = time_tuning(b, &t, repeats, err, sizeof(err));
+      if (ms < 0) {
+        set_last_error(err);
+        ecp_dev_set_tuning(b->dev, &best);
+        return -1;
+      }
+      if (report && used < (size_t)len)
+        used += (size_t)snprintf(report + used, (size_t)len - used, "; %s=%d %.4f ms", space[k].name,
+                                 space[k].v[i], ms);
+      if (ms < 0.99 * tbest) {
+        tbest = ms;
+        best = t;
+      }
+    }
+  }
+  ecp_dev_set_tuning(b->dev, &best);
+  if (report && used < (size_t)len)
+    snprintf(report + used, (size_t)len - used, "; best %.4f ms: t1_coop_max=%d t1_head_coop=%d t1_head_serial=%d "
+             "t1_tail_grid=%d", tbest, best.t1_coop_max, best.t1_head_coop, best.t1_head_serial, best.t1_tail_grid);
+  return 0;
+}

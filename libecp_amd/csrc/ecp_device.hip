@@ -380,111 +380,268 @@ __device__ __forceinline__ void t1_geom(const DevJob &J, const ecp_triplet &tr, 
   G.z = -G.za - G.zb;
 }
 
-template <int LAB>
-__global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Unit *__restrict__ units, int nunits,
-                                              double *__restrict__ qout, unsigned long long *__restrict__ qfail,
-                                              int *__restrict__ fbq, int *__restrict__ fbcount) {
-  constexpr int NQ = nq_of(LAB);
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= nunits) return;
-  const T1Unit U = units[u];
-  const ecp_triplet tr = J.trip[U.trip];
-  const ecp_centre Cn = J.cent[tr.centre];
-  T1Geom G;
-  t1_geom(J, tr, U.pa, U.pb, G);
-  const double C0 = G.ca * G.cb * exp(G.zd2);
-  const double *__restrict__ UL = J.UL + Cn.UL_off;
-  const int gs = tr.gs, ge = tr.ge;
-  const double minExp = T.lnAcc1;
-
-  double p[NQ], q[NQ], I[NQ], Tt[NQ];
-  // w_n f_e(n) accumulated into acc[] (type1.c:78-88 integrand, tabulated only
-  // on [gs, ge) as type1.c:121-130)
-  auto accum = [&](int n, double (&acc)[NQ]) {
-    if (n < gs || n >= ge) return;
-    const double r = T.xs[n];
-    const double expo = (G.z * r + G.s) * r;
-    if (!(expo >= minExp)) return;
-    const double g = exp(expo);
-    double K[LAB + 1];
-    bessel_k<LAB>(T.bessel, G.s * r, K);
-    const double Un = UL[n], wn = T.ws[n];
-    double rn = 1.0;
-    int e = 0;
-#pragma unroll
-    for (int l1 = 0; l1 <= LAB; l1++) {
-      const double a = (C0 * rn) * Un;
-#pragma unroll
-      for (int l2 = l1; l2 >= 0; l2 -= 2, e++) acc[e] += wn * ((a * K[l2]) * g);
-      rn *= r;
-    }
-  };
-#pragma unroll
-  for (int e = 0; e < NQ; e++) Tt[e] = 0.0;
-  accum(T.s93.centre, Tt);
-#pragma unroll
-  for (int e = 0; e < NQ; e++) {
-    p[e] = Tt[e];
-    Tt[e] = 0.0;
-  }
-  accum(T.s93.first_l, Tt);
-  {
-    double t2[NQ];
-#pragma unroll
-    for (int e = 0; e < NQ; e++) t2[e] = 0.0;
-    accum(T.s93.first_r, t2);
-#pragma unroll
-    for (int e = 0; e < NQ; e++) {
-      q[e] = Tt[e] + t2[e];
-      I[e] = p[e] + q[e];
-    }
-  }
-  unsigned long long done = 0ull;
+// Results of a finished primitive pair: converged entries, zeros for the
+// others, which go to the large-grid fallback queue.
+template <int NQ>
+__device__ __forceinline__ void t1_finish(int u, const T1Unit &U, const double (&p)[NQ], unsigned long long done,
+                                          double *__restrict__ qout, unsigned long long *__restrict__ qfail,
+                                          int *__restrict__ fbq, int *__restrict__ fbcount) {
   const unsigned long long all = (NQ >= 64) ? ~0ull : ((1ull << NQ) - 1ull);
-  for (int lv = 2; lv < T.s93.nlevels; lv++) {
-    const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
-    int cnt = 0;
-    for (int k = T.s93.lvl_begin[lv]; k < T.s93.lvl_begin[lv + 1]; k++) {
-      const int li = T.p93l[k], ri = T.p93r[k];
-#pragma unroll
-      for (int e = 0; e < NQ; e++) Tt[e] = 0.0;
-      if (li >= gs) {
-        cnt++;
-        accum(li, Tt);
-      }
-      if (ri <= ge) {
-        cnt++;
-        accum(ri, Tt);
-      }
-#pragma unroll
-      for (int e = 0; e < NQ; e++) I[e] += Tt[e];
-    }
-#pragma unroll
-    for (int e = 0; e < NQ; e++) {
-      if (done & (1ull << e)) continue;
-      double err;
-      if (j == 0) {
-        p[e] += (I[e] - q[e]);
-        err = 16 * fabs(q[e] - 3 * p[e] / 2) / (3 * nn);
-      } else {
-        err = 16 * fabs(I[e] - 2 * q[e]) / (3 * nn);
-        q[e] = I[e];
-      }
-      if (cnt > 0 && err < T.tol) {
-        done |= (1ull << e);
-        p[e] = 16 * q[e] / (3 * nn);
-      }
-    }
-    if (done == all) break;
-  }
   double *out = qout + U.qoff;
 #pragma unroll
   for (int e = 0; e < NQ; e++) out[e] = (done & (1ull << e)) ? p[e] : 0.0;
   const unsigned long long failed = all & ~done;
   qfail[u] = failed;
-  if (failed) {
-    const int slot = atomicAdd(fbcount, 1);
-    fbq[slot] = u;
+  if (failed) fbq[atomicAdd(fbcount, 1)] = u;
+}
+
+// Type-1 small grid, one lane per primitive pair (MODE 0: the whole PS93
+// sequence; MODE 1, head: levels [0, head), then a pair still open saves its
+// integrator state -- p, q, I per entry in planes of `plane` doubles at
+// state -- and its converged-entry mask and is queued for a tail kernel;
+// MODE 2, lane-serial tail: the queued pairs, compacted one per lane, resume
+// at level `head`).  Within a level each point pair's sum
+// T = (0 + f(left)) + f(right) is added to I in pair order, and the PS93
+// recurrences and test (gc_integrators.c:156-217) follow.
+template <int LAB, int MODE>
+__global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Unit *__restrict__ units, int nunits,
+                                              double *__restrict__ qout, unsigned long long *__restrict__ qfail,
+                                              int *__restrict__ fbq, int *__restrict__ fbcount, int head,
+                                              double *__restrict__ state, size_t plane, int *__restrict__ tq,
+                                              int *__restrict__ tcount) {
+  constexpr int NQ = nq_of(LAB);
+  const int count = MODE == 2 ? *tcount : nunits;
+  const int stride = MODE == 2 ? gridDim.x * blockDim.x : count;
+  for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < count; it += stride) {
+    const int u = MODE == 2 ? tq[it] : it;
+    const T1Unit U = units[u];
+    const ecp_triplet tr = J.trip[U.trip];
+    const ecp_centre Cn = J.cent[tr.centre];
+    T1Geom G;
+    t1_geom(J, tr, U.pa, U.pb, G);
+    const double C0 = G.ca * G.cb * exp(G.zd2);
+    const double *__restrict__ UL = J.UL + Cn.UL_off;
+    const int gs = tr.gs, ge = tr.ge;
+    const double minExp = T.lnAcc1;
+    // w_n f(n) of one point for every entry, added into acc[] (type1.c:78-88
+    // integrand, tabulated only on [gs, ge) as type1.c:121-130)
+    auto f = [&](int n, double (&acc)[NQ]) {
+      if (n < gs || n >= ge) return;
+      const double r = T.xs[n];
+      const double expo = (G.z * r + G.s) * r;
+      if (!(expo >= minExp)) return;
+      const double g = exp(expo);
+      double K[LAB + 1];
+      bessel_k<LAB>(T.bessel, G.s * r, K);
+      const double Un = UL[n], wn = T.ws[n];
+      double rn = 1.0;
+      int e = 0;
+#pragma unroll
+      for (int l1 = 0; l1 <= LAB; l1++) {
+        const double a = (C0 * rn) * Un;
+#pragma unroll
+        for (int l2 = l1; l2 >= 0; l2 -= 2, e++) acc[e] += wn * ((a * K[l2]) * g);
+        rn *= r;
+      }
+    };
+    double p[NQ], q[NQ], I[NQ], Tt[NQ];
+    unsigned long long done = 0ull;
+    const unsigned long long all = (NQ >= 64) ? ~0ull : ((1ull << NQ) - 1ull);
+    int lv0 = 2, lv1 = T.s93.nlevels;
+    if (MODE == 2) {
+#pragma unroll
+      for (int e = 0; e < NQ; e++) {
+        p[e] = state[U.qoff + e];
+        q[e] = state[plane + U.qoff + e];
+        I[e] = state[2 * plane + U.qoff + e];
+      }
+      done = qfail[u];
+      lv0 = head;
+    } else {
+#pragma unroll
+      for (int e = 0; e < NQ; e++) Tt[e] = 0.0;
+      f(T.s93.centre, Tt);
+#pragma unroll
+      for (int e = 0; e < NQ; e++) {
+        p[e] = Tt[e];
+        Tt[e] = 0.0;
+      }
+      f(T.s93.first_l, Tt);
+      double t2[NQ];
+#pragma unroll
+      for (int e = 0; e < NQ; e++) t2[e] = 0.0;
+      f(T.s93.first_r, t2);
+#pragma unroll
+      for (int e = 0; e < NQ; e++) {
+        q[e] = Tt[e] + t2[e];
+        I[e] = p[e] + q[e];
+      }
+      if (MODE == 1) lv1 = min(head, T.s93.nlevels);
+    }
+    for (int lv = lv0; lv < lv1; lv++) {
+      const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
+      int cnt = 0;
+      for (int k = T.s93.lvl_begin[lv]; k < T.s93.lvl_begin[lv + 1]; k++) {
+        const int li = T.p93l[k], ri = T.p93r[k];
+#pragma unroll
+        for (int e = 0; e < NQ; e++) Tt[e] = 0.0;
+        if (li >= gs) {
+          cnt++;
+          f(li, Tt);
+        }
+        if (ri <= ge) {
+          cnt++;
+          f(ri, Tt);
+        }
+#pragma unroll
+        for (int e = 0; e < NQ; e++) I[e] += Tt[e];
+      }
+#pragma unroll
+      for (int e = 0; e < NQ; e++) {
+        if (done & (1ull << e)) continue;
+        double err;
+        if (j == 0) {
+          p[e] += (I[e] - q[e]);
+          err = 16 * fabs(q[e] - 3 * p[e] / 2) / (3 * nn);
+        } else {
+          err = 16 * fabs(I[e] - 2 * q[e]) / (3 * nn);
+          q[e] = I[e];
+        }
+        if (cnt > 0 && err < T.tol) {
+          done |= (1ull << e);
+          p[e] = 16 * q[e] / (3 * nn);
+        }
+      }
+      if (done == all) break;
+    }
+    if (MODE == 1 && done != all && lv1 < T.s93.nlevels) {
+#pragma unroll
+      for (int e = 0; e < NQ; e++) {
+        state[U.qoff + e] = p[e];
+        state[plane + U.qoff + e] = q[e];
+        state[2 * plane + U.qoff + e] = I[e];
+      }
+      qfail[u] = done;
+      tq[atomicAdd(tcount, 1)] = u;
+      continue;
+    }
+    t1_finish<NQ>(u, U, p, done, qout, qfail, fbq, fbcount);
+  }
+}
+
+// Wave-level helpers (blocks of exactly one 64-lane wave).
+__device__ __forceinline__ int wave_min(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// Tail: one wave per primitive pair left open by the head, levels [head,
+// nlevels).  Lane k evaluates the k-th point pair of the level (at most 64
+// pairs per level of the 383-point grid) for every Q entry and stores the
+// pair sum T_k = (0 + f(left)) + f(right) in LDS; lane e then adds the T_k
+// to its entry's I in pair order and applies the PS93 test, so the
+// arithmetic is that of the lane-serial integrator.
+constexpr int kTS = 65;  // LDS stride of the per-entry pair sums (bank spread)
+
+template <int LAB>
+__global__ __launch_bounds__(64) void k_t1tail(DevTables T, DevJob J, const T1Unit *__restrict__ units,
+                                               double *__restrict__ qout, unsigned long long *__restrict__ qfail,
+                                               int *__restrict__ fbq, int *__restrict__ fbcount, int head,
+                                               const double *__restrict__ state, size_t plane,
+                                               const int *__restrict__ tq, const int *__restrict__ tcount) {
+  constexpr int NQ = nq_of(LAB);
+  static_assert(NQ <= 64, "one lane per Q entry");
+  __shared__ double s_T[NQ * kTS];
+  const int count = *tcount;
+  const int lane = threadIdx.x;
+  for (int it = blockIdx.x; it < count; it += gridDim.x) {
+    const int u = tq[it];
+    const T1Unit U = units[u];
+    const ecp_triplet tr = J.trip[U.trip];
+    const ecp_centre Cn = J.cent[tr.centre];
+    T1Geom G;
+    t1_geom(J, tr, U.pa, U.pb, G);
+    const double C0 = G.ca * G.cb * exp(G.zd2);
+    const double *__restrict__ UL = J.UL + Cn.UL_off;
+    const int gs = tr.gs, ge = tr.ge;
+    const double minExp = T.lnAcc1;
+    auto f = [&](int n, double (&acc)[NQ]) {
+      if (n < gs || n >= ge) return;
+      const double r = T.xs[n];
+      const double expo = (G.z * r + G.s) * r;
+      if (!(expo >= minExp)) return;
+      const double g = exp(expo);
+      double K[LAB + 1];
+      bessel_k<LAB>(T.bessel, G.s * r, K);
+      const double Un = UL[n], wn = T.ws[n];
+      double rn = 1.0;
+      int e = 0;
+#pragma unroll
+      for (int l1 = 0; l1 <= LAB; l1++) {
+        const double a = (C0 * rn) * Un;
+#pragma unroll
+        for (int l2 = l1; l2 >= 0; l2 -= 2, e++) acc[e] += wn * ((a * K[l2]) * g);
+        rn *= r;
+      }
+    };
+    const unsigned long long done0 = qfail[u];
+    double p = 0.0, q = 0.0, I = 0.0;
+    bool mydone = true;
+    if (lane < NQ) {
+      p = state[U.qoff + lane];
+      q = state[plane + U.qoff + lane];
+      I = state[2 * plane + U.qoff + lane];
+      mydone = (done0 >> lane) & 1ull;
+    }
+    for (int lv = head; lv < T.s93.nlevels; lv++) {
+      const int b0 = T.s93.lvl_begin[lv], np = T.s93.lvl_begin[lv + 1] - b0;
+      int cnt = 0;
+      if (lane < np) {
+        const int li = T.p93l[b0 + lane], ri = T.p93r[b0 + lane];
+        double t[NQ];
+#pragma unroll
+        for (int e = 0; e < NQ; e++) t[e] = 0.0;
+        if (li >= gs) {
+          cnt++;
+          f(li, t);
+        }
+        if (ri <= ge) {
+          cnt++;
+          f(ri, t);
+        }
+#pragma unroll
+        for (int e = 0; e < NQ; e++) s_T[e * kTS + lane] = t[e];
+      }
+      const bool any = wave_any(cnt > 0);
+      __syncthreads();
+      if (lane < NQ)
+        for (int k = 0; k < np; k++) I += s_T[lane * kTS + k];
+      __syncthreads();
+      if (!mydone) {
+        const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
+        double err;
+        if (j == 0) {
+          p += (I - q);
+          err = 16 * fabs(q - 3 * p / 2) / (3 * nn);
+        } else {
+          err = 16 * fabs(I - 2 * q) / (3 * nn);
+          q = I;
+        }
+        if (any && err < T.tol) {
+          mydone = true;
+          p = 16 * q / (3 * nn);
+        }
+      }
+      if (!wave_any(!mydone)) break;
+    }
+    if (lane < NQ) qout[U.qoff + lane] = mydone ? p : 0.0;
+    const unsigned long long failed = __ballot(lane < NQ && !mydone);
+    if (lane == 0) {
+      qfail[u] = failed;
+      if (failed) fbq[atomicAdd(fbcount, 1)] = u;
+    }
   }
 }
 
@@ -502,14 +659,6 @@ __device__ __forceinline__ Fm06 fm06(double zeta_p, double P) {
   m.i2 = 0.5 * (rmax + rmin);
   return m;
 }
-
-// Wave-level helpers (blocks of exactly one 64-lane wave).
-__device__ __forceinline__ int wave_min(int v) {
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-  return v;
-}
-
-__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
 // First large-grid index n with r_n > max(dAC, dBC) and the exponent below
 // the numerical zero: the screening break of type1.c:222-225 /
@@ -1128,7 +1277,7 @@ struct JobLayout {
   size_t i_atoms = 0, i_amaxl = 0, i_sh = 0, i_pa = 0, i_pd = 0, i_cent = 0, i_gauss = 0, i_win = 0, i_trip = 0,
          i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, in_bytes = 0;
   size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
-         d_q = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
+         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0;
 };
 
@@ -1145,6 +1294,9 @@ struct Prepared {
   DevJob J{};
   const T1Unit *u1 = nullptr;
   double *q = nullptr;
+  size_t nq = 0;
+  double *t1s = nullptr;  // type-1 head -> tail state: p, q, I planes of nq doubles
+  int *tq1 = nullptr;     // type-1 tail queues (per class, indexed like the units)
   unsigned long long *qf = nullptr;
   int *fbq1 = nullptr, *fbc = nullptr;
   unsigned char *stat = nullptr;
@@ -1190,6 +1342,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
+  ecp_tuning tune{8, 15, 100000, 8192};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1202,8 +1355,8 @@ struct ecp_dev {
 
 namespace {
 // kernel families reported by the profiler
-enum { FAM_TABLES = 0, FAM_TYPE1, FAM_TYPE2, FAM_FB1, FAM_FB2, FAM_ASM, FAM_N };
-const char *kFamName[FAM_N] = {"tables", "type1", "type2", "fallback1", "fallback2", "assemble"};
+enum { FAM_TABLES = 0, FAM_TYPE1, FAM_TYPE2, FAM_FB1, FAM_FB2, FAM_ASM, FAM_T1TAIL, FAM_N };
+const char *kFamName[FAM_N] = {"tables", "type1", "type2", "fallback1", "fallback2", "assemble", "type1_tail"};
 
 // records an event pair around one launch on stream s when profiling
 struct Span {
@@ -1410,13 +1563,37 @@ extern "C" int ecp_dev_last_timing(ecp_dev *d, double *ms, int n) {
 
 namespace {
 
+// Type-1 small grid of one class: the head kernel, then -- for the pairs
+// still open after `head` levels -- the wave-cooperative tail when the class
+// is small (latency-bound: few waves on the device) or the compacted
+// lane-serial tail when it is large (throughput-bound).
+// Type-1 small grid of one class: without a tail (head >= the level count)
+// one lane-serial kernel; otherwise the head kernel, then for the pairs still
+// open the wave-cooperative tail when the class is small (latency-bound: few
+// waves on the device) or the compacted lane-serial tail when it is large.
 template <int LAB>
 void launch_type1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T1Unit *u, int n, double *q,
-                  unsigned long long *qf, int *fbq, int *fbc) {
-  if (n > 0) {
+                  unsigned long long *qf, int *fbq, int *fbc, double *st, size_t plane, int *tq, int *tc) {
+  if (n <= 0) return;
+  const bool coop = n <= d->tune.t1_coop_max;
+  const int head = coop ? d->tune.t1_head_coop : d->tune.t1_head_serial;
+  const int grid = (n + 63) / 64;
+  if (head >= T.s93.nlevels) {
     Span sp(d, s, FAM_TYPE1);
-    k_type1<LAB><<<(n + 63) / 64, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc);
+    k_type1<LAB, 0><<<grid, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st, plane, tq, tc);
+    return;
   }
+  {
+    Span sp(d, s, FAM_TYPE1);
+    k_type1<LAB, 1><<<grid, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st, plane, tq, tc);
+  }
+  Span sp(d, s, FAM_T1TAIL);
+  if (coop)
+    k_t1tail<LAB><<<std::min(n, d->tune.t1_tail_grid), 64, 0, s>>>(T, J, u, q, qf, fbq, fbc, head, st, plane, tq,
+                                                                    tc);
+  else
+    k_type1<LAB, 2><<<std::min(grid, d->tune.t1_tail_grid), 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st,
+                                                                         plane, tq, tc);
 }
 
 template <int LAB>
@@ -1475,7 +1652,8 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   {                                                                                                             \
     hipStream_t ws = w[(c) % kWorkers];                                                                         \
     launch_type1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                \
-                    P.fbq1 + P.cls_begin[c], P.fbc + c);                                                        \
+                    P.fbq1 + P.cls_begin[c], P.fbc + c, P.t1s, P.nq, P.tq1 + P.cls_begin[c],                     \
+                    P.fbc + 32 + c);                                                                            \
     launch_fb1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                  \
                   P.fbq1 + P.cls_begin[c], P.fbc + c, P.stat);                                                  \
   }
@@ -1620,6 +1798,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_F = dv_take(8 * job->nF);
   Ly.d_ang = dv_take(8 * job->nang);
   Ly.d_q = dv_take(8 * nq);
+  P.nq = nq;
   Ly.d_qf = dv_take(8 * units1.size());
   Ly.d_fbq1 = dv_take(4 * units1.size());
   Ly.d_T = dv_take(8 * (size_t)ntot2);
@@ -1627,6 +1806,8 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_fbq2 = dv_take(4 * (size_t)(kMaxLMB + 1) * P.ngroups);
   Ly.d_fbT = dv_take(8 * (size_t)ntot2 * P.maxPairs);
   Ly.d_blk = dv_take(8 * job->nblk);
+  Ly.d_t1s = dv_take(3 * 8 * nq);
+  Ly.d_tq1 = dv_take(4 * units1.size());
   Ly.total = dv_off;
   Ly.FL = job->FL;
   Ly.UK = job->UK;
@@ -1748,9 +1929,11 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   J.maxPairs = P.maxPairs;
   P.u1 = (const T1Unit *)(A + Ly.i_u1);
   P.q = (double *)(A + Ly.d_q);
+  P.t1s = (double *)(A + Ly.d_t1s);
+  P.tq1 = (int *)(A + Ly.d_tq1);
   P.qf = (unsigned long long *)(A + Ly.d_qf);
   P.fbq1 = (int *)(A + Ly.d_fbq1);
-  P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 queues, [16..27] type-2 queues
+  P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 fallback queues, [16..27] type-2, [32..42] type-1 tails
   P.stat = (unsigned char *)(A + Ly.d_stat);
   P.grp = (const T2Group *)(A + Ly.i_grp);
   P.goff = (const int *)(A + Ly.i_goff);
@@ -1902,3 +2085,12 @@ extern "C" int ecp_dev_current(void) {
 }
 
 extern "C" int ecp_dev_use(int device) { return hipSetDevice(device) == hipSuccess ? 0 : -1; }
+
+extern "C" void ecp_dev_get_tuning(const ecp_dev *d, ecp_tuning *t) { *t = d->tune; }
+
+extern "C" void ecp_dev_set_tuning(ecp_dev *d, ecp_tuning const *t) {
+  d->tune = *t;
+  d->tune.t1_tail_grid = std::max(1, d->tune.t1_tail_grid);
+  reset_graph(d);
+  d->eager_done = false;
+}

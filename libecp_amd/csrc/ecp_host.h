@@ -68,6 +68,14 @@ typedef struct {
   int ang_lam, ang_c;        /* its lambda and Cartesian dimensions */
 } ecp_job;
 
+/* ---- launch parameters of the device pipeline (tunable per context) ---- */
+typedef struct {
+  int t1_head_coop;   /* PS93 levels of the type-1 head kernel before the wave-cooperative tail */
+  int t1_head_serial; /* ... before the lane-serial tail (>= the level count: no tail) */
+  int t1_coop_max;    /* largest class (primitive pairs) that takes the cooperative tail */
+  int t1_tail_grid;   /* most workgroups of a tail launch (persistent over its queue) */
+} ecp_tuning;
+
 /* ---- device layer (ecp_device.hip) ---- */
 typedef struct ecp_dev ecp_dev;
 
@@ -99,6 +107,10 @@ const char *ecp_dev_family_name(int f);
 /* Stage timings of the last execute (ms): tables, quadratures (small grid
    and large-grid fallbacks, concurrent), assembly[, mean graph replay]. */
 int ecp_dev_last_timing(ecp_dev *d, double *ms, int n);
+/* launch parameters: defaults at creation; setting them drops a captured
+   graph (the next execute runs eagerly and captures again) */
+void ecp_dev_get_tuning(const ecp_dev *d, ecp_tuning *t);
+void ecp_dev_set_tuning(ecp_dev *d, const ecp_tuning *t);
 /* device memory and the calling thread's current HIP device */
 int ecp_dev_mem_alloc(void **p, size_t bytes, char *err, size_t errlen);
 void ecp_dev_mem_free(void *p);

@@ -168,6 +168,22 @@ class Batch:
             raise RuntimeError("libecp_batch_profile failed: " + last_error())
         return {L.libecp_kernel_family(i).decode(): (la[i], ms[i], fl[i]) for i in range(k)}
 
+    def set_param(self, name, value):
+        if lib().libecp_batch_set_param(C.c_void_p(self.b), name.encode(), C.c_int(value)) != 0:
+            raise RuntimeError("libecp_batch_set_param failed: " + last_error())
+
+    def get_param(self, name):
+        v = C.c_int()
+        if lib().libecp_batch_get_param(C.c_void_p(self.b), name.encode(), C.byref(v)) != 0:
+            raise RuntimeError("libecp_batch_get_param failed: " + last_error())
+        return v.value
+
+    def autotune(self, repeats=3):
+        buf = C.create_string_buffer(4096)
+        if lib().libecp_batch_autotune(C.c_void_p(self.b), C.c_int(repeats), buf, C.c_int(4096)) != 0:
+            raise RuntimeError("libecp_batch_autotune failed: " + last_error())
+        return buf.value.decode()
+
     def close(self):
         if self.b:
             lib().libecp_batch_free(C.c_void_p(self.b))
