@@ -216,7 +216,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the launch-parameter autotuner on each job before timing (default: built-in choices)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)
@@ -240,7 +241,7 @@ def main():
     m = medium_cluster(0)
     b = native.Batch([m])
     nominal, evaluated = b.triplets()
-    tune_report = b.autotune(3) if not args.no_autotune else "defaults"
+    tune_report = b.autotune(3) if args.autotune else "defaults"
     b.execute(max(1, args.warmup))
     stages = b.stage_ms()
     prof = b.profile()
@@ -299,7 +300,7 @@ def main():
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": achieved / FP64_PEAK_TFLOPS,
                        "traffic": measured_traffic() if fam == "type1" else None,
-                       "kernel": "k_type1 + k_t1tail" if fam == "type1" else "k_type2",
+                       "kernel": "type-1 small grid (k_type1*, k_t1tail*)" if fam == "type1" else "k_type2",
                        "launches_per_step": launches,
                        "avg_launch_ms": kms / launches, "algorithmic_flop_per_launch": kflops / launches,
                        "note": "FP64 vector work (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); "
@@ -329,7 +330,7 @@ def main():
         bb = native.Batch(mols)
         t_create = time.perf_counter() - t
         nom_b, ev_b = bb.triplets()
-        tune_b = bb.autotune(2) if not args.no_autotune else "defaults"
+        tune_b = bb.autotune(2) if args.autotune else "defaults"
         bb.execute(2)
         st_b = bb.stage_ms()
         t = time.perf_counter()

@@ -69,8 +69,11 @@ const char *libecp_kernel_family(int i);
    "t1_head_coop" / "t1_head_serial" (PS93 levels the type-1 head kernel runs
    before the wave-cooperative / lane-serial tail), "t1_coop_max" (largest
    type-1 class, in primitive pairs, that takes the cooperative tail),
-   "t1_tail_grid" (most workgroups of a tail launch).  Results do not depend
-   on them.  0, or -1 for an unknown name. */
+   "t1_tail_grid" (most workgroups of a tail launch), "fuse" (1: one launch
+   per stage covering every class, with assembly overlapping the large-grid
+   fallbacks; 0: per-class launches on forked streams; 2, the default: fused
+   for small jobs).  Results do not depend on them.  0, or -1 for an unknown
+   name. */
 int libecp_batch_set_param(libecp_batch *b, const char *name, int value);
 int libecp_batch_get_param(libecp_batch *b, const char *name, int *value);
 /* Autotuner: times the batch (graph replays, `repeats` per candidate) over

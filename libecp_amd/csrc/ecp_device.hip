@@ -1,19 +1,21 @@
 // HIP device layer of the ECP integral engine, written for gfx950 (MI355X).
 //
-// Pipeline of one job (all FP64, one HIP stream):
+// Pipeline of one job (all FP64; a main stream forks into worker streams
+// and joins them, captured once into a HIP graph):
 //   k_utab   U_L(r_n) and r_n^k U_l(r_n) on the 383-point small grid, per centre
 //            (evalECP / tabECP_FM06: ecp.c:41-60, type2.c:184-219)
 //   k_ftab   F_lambda(r_n) per (centre, shell) on its screened window
 //            (calcF_FM06: type2.c:222-310)
 //   k_ang    per-(centre, atom) angular tables (evalAngularIntegrals:
 //            angular_integrals.c:104-142)
-//   k_type1  type-1 radial integrals Q[l1][l2] per (triplet, primitive pair)
-//            with the PS93 sequence (calcQ: type1.c:94-203)
+//   k_type1 / k_t1tail  type-1 radial integrals Q[l1][l2] per (triplet,
+//            primitive pair) with the PS93 sequence (calcQ: type1.c:94-203):
+//            lane-serial head, wave-cooperative tail for the open pairs
 //   k_type2  type-2 radial integrals T_l[l1][l2][l3] per entry with PS93
 //            (calcT_fast_FM06: type2.c:336-381)
 //   k_fb1 / k_fb2  PSM92 large-grid redo of failed entries on FM06-mapped
 //            grids (type1.c:205-252, calcT_FM06: type2.c:417-528)
-//   k_assemble  type-1 and type-2 blocks, binomial shift and norms
+//   k_asm1 / k_asm2  type-1 / type-2 blocks, binomial shift and norms
 //            (calcChi: type1.c:211-302, calcGamma: type2.c:532-631,
 //             calcPolynomials: util.c:246-334)
 //
@@ -385,14 +387,18 @@ __device__ __forceinline__ void t1_geom(const DevJob &J, const ecp_triplet &tr, 
 template <int NQ>
 __device__ __forceinline__ void t1_finish(int u, const T1Unit &U, const double (&p)[NQ], unsigned long long done,
                                           double *__restrict__ qout, unsigned long long *__restrict__ qfail,
-                                          int *__restrict__ fbq, int *__restrict__ fbcount) {
+                                          int *__restrict__ fbq, int *__restrict__ fbcount,
+                                          unsigned char *__restrict__ fbflag) {
   const unsigned long long all = (NQ >= 64) ? ~0ull : ((1ull << NQ) - 1ull);
   double *out = qout + U.qoff;
 #pragma unroll
   for (int e = 0; e < NQ; e++) out[e] = (done & (1ull << e)) ? p[e] : 0.0;
   const unsigned long long failed = all & ~done;
   qfail[u] = failed;
-  if (failed) fbq[atomicAdd(fbcount, 1)] = u;
+  if (failed) {
+    fbq[atomicAdd(fbcount, 1)] = u;
+    fbflag[U.trip] = 1;
+  }
 }
 
 // Type-1 small grid, one lane per primitive pair (MODE 0: the whole PS93
@@ -404,15 +410,16 @@ __device__ __forceinline__ void t1_finish(int u, const T1Unit &U, const double (
 // T = (0 + f(left)) + f(right) is added to I in pair order, and the PS93
 // recurrences and test (gc_integrators.c:156-217) follow.
 template <int LAB, int MODE>
-__global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Unit *__restrict__ units, int nunits,
-                                              double *__restrict__ qout, unsigned long long *__restrict__ qfail,
-                                              int *__restrict__ fbq, int *__restrict__ fbcount, int head,
-                                              double *__restrict__ state, size_t plane, int *__restrict__ tq,
-                                              int *__restrict__ tcount) {
+__device__ __forceinline__ void type1_body(int bid, int nb, const DevTables &T, const DevJob &J,
+                                           const T1Unit *__restrict__ units, int nunits, double *__restrict__ qout,
+                                           unsigned long long *__restrict__ qfail, int *__restrict__ fbq,
+                                           int *__restrict__ fbcount, int head, double *__restrict__ state,
+                                           size_t plane, int *__restrict__ tq, int *__restrict__ tcount,
+                                           unsigned char *__restrict__ fbflag) {
   constexpr int NQ = nq_of(LAB);
   const int count = MODE == 2 ? *tcount : nunits;
-  const int stride = MODE == 2 ? gridDim.x * blockDim.x : count;
-  for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < count; it += stride) {
+  const int stride = MODE == 2 ? nb * 64 : count;
+  for (int it = bid * 64 + (int)threadIdx.x; it < count; it += stride) {
     const int u = MODE == 2 ? tq[it] : it;
     const T1Unit U = units[u];
     const ecp_triplet tr = J.trip[U.trip];
@@ -525,8 +532,18 @@ __global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Uni
       tq[atomicAdd(tcount, 1)] = u;
       continue;
     }
-    t1_finish<NQ>(u, U, p, done, qout, qfail, fbq, fbcount);
+    t1_finish<NQ>(u, U, p, done, qout, qfail, fbq, fbcount, fbflag);
   }
+}
+
+template <int LAB, int MODE>
+__global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Unit *__restrict__ units, int nunits,
+                                              double *__restrict__ qout, unsigned long long *__restrict__ qfail,
+                                              int *__restrict__ fbq, int *__restrict__ fbcount, int head,
+                                              double *__restrict__ state, size_t plane, int *__restrict__ tq,
+                                              int *__restrict__ tcount, unsigned char *__restrict__ fbflag) {
+  type1_body<LAB, MODE>(blockIdx.x, gridDim.x, T, J, units, nunits, qout, qfail, fbq, fbcount, head, state, plane,
+                        tq, tcount, fbflag);
 }
 
 // Wave-level helpers (blocks of exactly one 64-lane wave).
@@ -546,17 +563,17 @@ __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 constexpr int kTS = 65;  // LDS stride of the per-entry pair sums (bank spread)
 
 template <int LAB>
-__global__ __launch_bounds__(64) void k_t1tail(DevTables T, DevJob J, const T1Unit *__restrict__ units,
-                                               double *__restrict__ qout, unsigned long long *__restrict__ qfail,
-                                               int *__restrict__ fbq, int *__restrict__ fbcount, int head,
-                                               const double *__restrict__ state, size_t plane,
-                                               const int *__restrict__ tq, const int *__restrict__ tcount) {
+__device__ __forceinline__ void t1tail_body(int bid, int nb, const DevTables &T, const DevJob &J,
+                                            const T1Unit *__restrict__ units, double *__restrict__ qout,
+                                            unsigned long long *__restrict__ qfail, int *__restrict__ fbq,
+                                            int *__restrict__ fbcount, int head, const double *__restrict__ state,
+                                            size_t plane, const int *__restrict__ tq, const int *__restrict__ tcount,
+                                            unsigned char *__restrict__ fbflag, double *s_T) {
   constexpr int NQ = nq_of(LAB);
   static_assert(NQ <= 64, "one lane per Q entry");
-  __shared__ double s_T[NQ * kTS];
   const int count = *tcount;
   const int lane = threadIdx.x;
-  for (int it = blockIdx.x; it < count; it += gridDim.x) {
+  for (int it = bid; it < count; it += nb) {
     const int u = tq[it];
     const T1Unit U = units[u];
     const ecp_triplet tr = J.trip[U.trip];
@@ -640,9 +657,24 @@ __global__ __launch_bounds__(64) void k_t1tail(DevTables T, DevJob J, const T1Un
     const unsigned long long failed = __ballot(lane < NQ && !mydone);
     if (lane == 0) {
       qfail[u] = failed;
-      if (failed) fbq[atomicAdd(fbcount, 1)] = u;
+      if (failed) {
+        fbq[atomicAdd(fbcount, 1)] = u;
+        fbflag[U.trip] = 1;
+      }
     }
   }
+}
+
+template <int LAB>
+__global__ __launch_bounds__(64) void k_t1tail(DevTables T, DevJob J, const T1Unit *__restrict__ units,
+                                               double *__restrict__ qout, unsigned long long *__restrict__ qfail,
+                                               int *__restrict__ fbq, int *__restrict__ fbcount, int head,
+                                               const double *__restrict__ state, size_t plane,
+                                               const int *__restrict__ tq, const int *__restrict__ tcount,
+                                               unsigned char *__restrict__ fbflag) {
+  __shared__ double s_T[nq_of(LAB) * kTS];
+  t1tail_body<LAB>(blockIdx.x, gridDim.x, T, J, units, qout, qfail, fbq, fbcount, head, state, plane, tq, tcount,
+                   fbflag, s_T);
 }
 
 // FM06 linear map of the large grid for zeta_p, P (gc_integrators.c:316-331)
@@ -689,14 +721,14 @@ __device__ int fm06_break(const DevTables &T, const Fm06 &M, double dAC, double 
 constexpr int kKS = 65;  // LDS stride of per-point Bessel rows (bank spread)
 
 template <int LAB>
-__global__ __launch_bounds__(64) void k_fb1(DevTables T, DevJob J, const T1Unit *__restrict__ units,
-                                            double *__restrict__ qout, const unsigned long long *__restrict__ qfail,
-                                            const int *__restrict__ fbq, const int *__restrict__ fbcount,
-                                            unsigned char *__restrict__ status) {
-  __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(LAB + 1) * kKS];
+__device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, const DevJob &J,
+                                         const T1Unit *__restrict__ units, double *__restrict__ qout,
+                                         const unsigned long long *__restrict__ qfail, const int *__restrict__ fbq,
+                                         const int *__restrict__ fbcount, unsigned char *__restrict__ status,
+                                         double *s_r, double *s_w, double *s_u, double *s_g, double *s_k) {
   const int count = *fbcount;
   const int lane = threadIdx.x;
-  for (int it = blockIdx.x; it < count; it += gridDim.x) {
+  for (int it = bid; it < count; it += nb) {
     const int u = fbq[it];
     const T1Unit U = units[u];
     const unsigned long long mask = qfail[u];
@@ -797,6 +829,15 @@ __global__ __launch_bounds__(64) void k_fb1(DevTables T, DevJob J, const T1Unit 
   }
 }
 
+template <int LAB>
+__global__ __launch_bounds__(64) void k_fb1(DevTables T, DevJob J, const T1Unit *__restrict__ units,
+                                            double *__restrict__ qout, const unsigned long long *__restrict__ qfail,
+                                            const int *__restrict__ fbq, const int *__restrict__ fbcount,
+                                            unsigned char *__restrict__ status) {
+  __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(LAB + 1) * kKS];
+  fb1_body<LAB>(blockIdx.x, gridDim.x, T, J, units, qout, qfail, fbq, fbcount, status, s_r, s_w, s_u, s_g, s_k);
+}
+
 // ---------------------------------------------------------------------------
 // type 2: one lane per T entry
 
@@ -818,7 +859,8 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Gr
                                                const int *__restrict__ goff, int ngroups, int ntotal,
                                                const int *__restrict__ tmpl, double *__restrict__ Tout,
                                                unsigned char *__restrict__ tfail, int *__restrict__ gflag,
-                                               int *__restrict__ fbq, int *__restrict__ fbcount, int qstride) {
+                                               int *__restrict__ fbq, int *__restrict__ fbcount, int qstride,
+                                               unsigned char *__restrict__ fbflag) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= ntotal) return;
   const int g = find_group(goff, ngroups, gid);
@@ -872,6 +914,7 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Gr
   if (!ok && atomicOr(&gflag[g], 1) == 0) {
     const int slot = atomicAdd(&fbcount[G.lmb], 1);
     fbq[(size_t)G.lmb * qstride + slot] = g;
+    fbflag[G.trip] = 1;
   }
 }
 
@@ -881,19 +924,18 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Gr
 // entries (up to 4 each, 256 per pass) and sum the points in the
 // integrator's pair order.  Results per (entry, pair) go to fbT and are summed
 // over pairs, in pair order, by the assembly.
-constexpr int kFb2PerLane = 4;
+constexpr int kFb2PerLane = 1;
 
 template <int LMB>
-__global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group *__restrict__ groups,
-                                            const int *__restrict__ tmpl, const unsigned char *__restrict__ tfail,
-                                            const int *__restrict__ fbq, const int *__restrict__ fbcount,
-                                            double *__restrict__ fbT, unsigned char *__restrict__ status) {
-  __shared__ double s_r[64], s_w[64], s_cu[64], s_ex[64];
-  __shared__ double s_ka[(LMB + 1) * kKS], s_kb[(LMB + 1) * kKS];
-  __shared__ int s_ent[64 * kFb2PerLane];
+__device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, const DevJob &J,
+                                         const T2Group *__restrict__ groups, const int *__restrict__ tmpl,
+                                         const unsigned char *__restrict__ tfail, const int *__restrict__ fbq,
+                                         const int *__restrict__ fbcount, double *__restrict__ fbT,
+                                         unsigned char *__restrict__ status, double *s_r, double *s_w, double *s_cu,
+                                         double *s_ex, double *s_ka, double *s_kb, int *s_ent) {
   const int lane = threadIdx.x;
   const long total = (long)(*fbcount) * J.maxPairs;
-  for (long it = blockIdx.x; it < total; it += gridDim.x) {
+  for (long it = bid; it < total; it += nb) {
     const int g = fbq[it / J.maxPairs];
     const int pair = (int)(it % J.maxPairs);
     const T2Group G = groups[g];
@@ -920,16 +962,13 @@ __global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group
     const ecp_gauss *gg = J.gauss + Cn.g0;
     auto eval = [&](int slot, int n) {
       double r = 0.0, w = 0.0, cu = 0.0, ex = 0.0;
-      double Ka[LMB + 1], Kb[LMB + 1];
-#pragma unroll
-      for (int l = 0; l <= LMB; l++) Ka[l] = Kb[l] = 0.0;
+      bool live = false;
       if (n >= 0 && n < brk) {
         r = M.i1 * T.xl[n] + M.i2;
         const double expo = expo_of(r);
         if (expo >= minExp) {
+          live = true;
           cu = Cc * eval_ecp(gg, Cn.ng, G.l, r);
-          bessel_k<LMB>(T.bessel, s1 * r, Ka);
-          bessel_k<LMB>(T.bessel, s2 * r, Kb);
           ex = exp(expo);
           w = T.wl[n] * M.i1;
         }
@@ -938,10 +977,24 @@ __global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group
       s_w[slot] = w;
       s_cu[slot] = cu;
       s_ex[slot] = ex;
+      // one Bessel vector live at a time (register pressure)
+      {
+        double K[LMB + 1];
 #pragma unroll
-      for (int l = 0; l <= LMB; l++) {
-        if (l <= laC) s_ka[l * kKS + slot] = Ka[l];
-        if (l <= lbC) s_kb[l * kKS + slot] = Kb[l];
+        for (int l = 0; l <= LMB; l++) K[l] = 0.0;
+        if (live) bessel_k<LMB>(T.bessel, s1 * r, K);
+#pragma unroll
+        for (int l = 0; l <= LMB; l++)
+          if (l <= laC) s_ka[l * kKS + slot] = K[l];
+      }
+      {
+        double K[LMB + 1];
+#pragma unroll
+        for (int l = 0; l <= LMB; l++) K[l] = 0.0;
+        if (live) bessel_k<LMB>(T.bessel, s2 * r, K);
+#pragma unroll
+        for (int l = 0; l <= LMB; l++)
+          if (l <= lbC) s_kb[l * kKS + slot] = K[l];
       }
     };
     // failed entries of the group, in entry order, processed 256 per pass
@@ -1040,6 +1093,136 @@ __global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group
   }
 }
 
+template <int LMB>
+__global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group *__restrict__ groups,
+                                            const int *__restrict__ tmpl, const unsigned char *__restrict__ tfail,
+                                            const int *__restrict__ fbq, const int *__restrict__ fbcount,
+                                            double *__restrict__ fbT, unsigned char *__restrict__ status) {
+  __shared__ double s_r[64], s_w[64], s_cu[64], s_ex[64];
+  __shared__ double s_ka[(LMB + 1) * kKS], s_kb[(LMB + 1) * kKS];
+  __shared__ int s_ent[64 * kFb2PerLane];
+  fb2_body<LMB>(blockIdx.x, gridDim.x, T, J, groups, tmpl, tfail, fbq, fbcount, fbT, status, s_r, s_w, s_cu, s_ex,
+                s_ka, s_kb, s_ent);
+}
+
+// ---------------------------------------------------------------------------
+// Fused launches: one grid covers every class (lambda = la + lb for type 1,
+// lambda_max = max(la, lb) + l for the type-2 fallback), each class owning a
+// contiguous range of workgroups; the class bodies are those of the
+// per-class kernels.  For small jobs, where each per-class launch holds only
+// a few waves, this turns ~40 dependent launches into a handful.
+
+struct ClassMap {
+  int n;                     // classes with work
+  int cls[16];               // class id of entry i
+  int bbeg[17];              // workgroup range of entry i: [bbeg[i], bbeg[i+1])
+};
+
+__device__ __forceinline__ int class_entry(const ClassMap &cm, int b) {
+  int i = 0;
+  while (i + 1 < cm.n && b >= cm.bbeg[i + 1]) i++;
+  return i;
+}
+
+// per-class views of the type-1 work arrays
+struct T1Set {
+  const T1Unit *units;
+  double *q;
+  unsigned long long *qf;
+  int *fbq, *fbc, *tq;
+  double *st;
+  size_t plane;
+  unsigned char *stat, *fbflag;
+  int ubeg[ECP_MAX_LAB + 1], ucnt[ECP_MAX_LAB + 1];
+  int head[ECP_MAX_LAB + 1];  // PS93 levels of the head per class (>= the level count: no tail)
+};
+
+#define ECP_CASES_LAB(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
+#define ECP_CASES_LMB(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
+
+template <int MAXLAB, int MODE>
+__global__ __launch_bounds__(64) void k_type1_fused(DevTables T, DevJob J, T1Set S, ClassMap cm) {
+  const int i = class_entry(cm, blockIdx.x);
+  const int c = cm.cls[i], bid = blockIdx.x - cm.bbeg[i], nb = cm.bbeg[i + 1] - cm.bbeg[i];
+  const int ub = S.ubeg[c];
+  switch (c) {
+#define X(L)                                                                                                   \
+  case L:                                                                                                      \
+    if constexpr (L <= MAXLAB)                                                                                 \
+      type1_body<L, MODE>(bid, nb, T, J, S.units + ub, S.ucnt[c], S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.head[c], \
+                          S.st, S.plane, S.tq + ub, S.fbc + 32 + c, S.fbflag);                                  \
+    break;
+    ECP_CASES_LAB(X)
+#undef X
+  }
+}
+
+template <int MAXLAB>
+__global__ __launch_bounds__(64) void k_t1tail_fused(DevTables T, DevJob J, T1Set S, ClassMap cm) {
+  __shared__ double s_T[nq_of(MAXLAB) * kTS];
+  const int i = class_entry(cm, blockIdx.x);
+  const int c = cm.cls[i], bid = blockIdx.x - cm.bbeg[i], nb = cm.bbeg[i + 1] - cm.bbeg[i];
+  const int ub = S.ubeg[c];
+  switch (c) {
+#define X(L)                                                                                                   \
+  case L:                                                                                                      \
+    if constexpr (L <= MAXLAB)                                                                                 \
+      t1tail_body<L>(bid, nb, T, J, S.units + ub, S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.head[c], S.st, S.plane, \
+                     S.tq + ub, S.fbc + 32 + c, S.fbflag, s_T);                                                 \
+    break;
+    ECP_CASES_LAB(X)
+#undef X
+  }
+}
+
+template <int MAXLAB>
+__global__ __launch_bounds__(64) void k_fb1_fused(DevTables T, DevJob J, T1Set S, ClassMap cm) {
+  __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(MAXLAB + 1) * kKS];
+  const int i = class_entry(cm, blockIdx.x);
+  const int c = cm.cls[i], bid = blockIdx.x - cm.bbeg[i], nb = cm.bbeg[i + 1] - cm.bbeg[i];
+  const int ub = S.ubeg[c];
+  switch (c) {
+#define X(L)                                                                                                   \
+  case L:                                                                                                      \
+    if constexpr (L <= MAXLAB)                                                                                 \
+      fb1_body<L>(bid, nb, T, J, S.units + ub, S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.stat, s_r, s_w, s_u, s_g, \
+                  s_k);                                                                                         \
+    break;
+    ECP_CASES_LAB(X)
+#undef X
+  }
+}
+
+struct T2Set {
+  const T2Group *groups;
+  const int *tmpl;
+  const unsigned char *tf;
+  const int *fbq2;
+  const int *fbc;
+  double *fbT;
+  unsigned char *stat;
+  int ngroups;
+};
+
+template <int MAXLMB>
+__global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S, ClassMap cm) {
+  __shared__ double s_r[64], s_w[64], s_cu[64], s_ex[64];
+  __shared__ double s_ka[(MAXLMB + 1) * kKS], s_kb[(MAXLMB + 1) * kKS];
+  __shared__ int s_ent[64 * kFb2PerLane];
+  const int i = class_entry(cm, blockIdx.x);
+  const int c = cm.cls[i], bid = blockIdx.x - cm.bbeg[i], nb = cm.bbeg[i + 1] - cm.bbeg[i];
+  switch (c) {
+#define X(L)                                                                                                   \
+  case L:                                                                                                      \
+    if constexpr (L <= MAXLMB)                                                                                 \
+      fb2_body<L>(bid, nb, T, J, S.groups, S.tmpl, S.tf, S.fbq2 + (size_t)L * S.ngroups, S.fbc + 16 + L, S.fbT,   \
+                  S.stat, s_r, s_w, s_cu, s_ex, s_ka, s_kb, s_ent);                                             \
+    break;
+    ECP_CASES_LMB(X)
+#undef X
+  }
+}
+
 // ---------------------------------------------------------------------------
 // assembly: type-1 block (calcChi), type-2 block (calcGamma), binomial shift
 // (calcPolynomials) and norms (libecp.c:234-235)
@@ -1058,16 +1241,21 @@ struct AsmArgs {
   const double *fbT;
   const double *fac, *dfac;
   double *blocks;
-  // dynamic LDS layout (doubles)
+  // dynamic LDS layout (doubles): J is shared; the type-1 region (rsph, H)
+  // and the type-2 region (blk, T) overlap
   int lds_blk, lds_H, lds_J, lds_rsph, lds_T, lds_rs_stride, pair_chunk;
 };
 
 // binomial shift of a block over C-centred monomials to the IJK(la) x IJK(lb)
 // output (util.c:246-334).  The block is read through get(r, q): row r is an
 // ordering position of degree <= la, column q a position of degree <= lb.
+struct V3 {
+  double x, y, z;
+};
+
 template <typename Get>
-__device__ void shift_block(const DevTables &T, Get get, int la, int lb, const double rA[3], const double rB[3],
-                            double norm, double *Jbuf, double *out) {
+__device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V3 rA, const V3 rB, double norm,
+                            double *Jbuf, double *out) {
   const int D = T.ijkDim;
   const int ncb = ECP_DIM_C(lb), nia = ECP_DIM_IJK(la), nib = ECP_DIM_IJK(lb);
   const double tol = T.accuracy;
@@ -1079,15 +1267,15 @@ __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const d
     for (int alx = 0; alx <= ax; alx++) {
       // (C - A)^(a - alpha) as unitarySpherePolynomials (util.c:214-243)
       double x = 1.0;
-      for (int i = 0; i < ax - alx; i++) x = -x * rA[0];
+      for (int i = 0; i < ax - alx; i++) x = -x * rA.x;
       const double bx = T.binom[ax * 16 + alx];
       for (int aly = 0; aly <= ay; aly++) {
         double y = 1.0;
-        for (int i = 0; i < ay - aly; i++) y = -y * rA[1];
+        for (int i = 0; i < ay - aly; i++) y = -y * rA.y;
         const double by = bx * T.binom[ay * 16 + aly];
         for (int alz = 0; alz <= az; alz++) {
           double z = 1.0;
-          for (int i = 0; i < az - alz; i++) z = -z * rA[2];
+          for (int i = 0; i < az - alz; i++) z = -z * rA.z;
           const double bz = by * T.binom[az * 16 + alz];
           const double factor = bz * ((x * y) * z);
           if (fabs(factor) <= tol) continue;
@@ -1105,15 +1293,15 @@ __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const d
     double acc = 0.0;
     for (int btx = 0; btx <= bx_; btx++) {
       double x = 1.0;
-      for (int i = 0; i < bx_ - btx; i++) x = -x * rB[0];
+      for (int i = 0; i < bx_ - btx; i++) x = -x * rB.x;
       const double fx = T.binom[bx_ * 16 + btx];
       for (int bty = 0; bty <= by_; bty++) {
         double y = 1.0;
-        for (int i = 0; i < by_ - bty; i++) y = -y * rB[1];
+        for (int i = 0; i < by_ - bty; i++) y = -y * rB.y;
         const double fy = fx * T.binom[by_ * 16 + bty];
         for (int btz = 0; btz <= bz_; btz++) {
           double z = 1.0;
-          for (int i = 0; i < bz_ - btz; i++) z = -z * rB[2];
+          for (int i = 0; i < bz_ - btz; i++) z = -z * rB.z;
           const double fz = fy * T.binom[bz_ * 16 + btz];
           double factor = fz * ((x * y) * z);
           if (fabs(factor) <= tol) continue;
@@ -1128,18 +1316,24 @@ __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const d
   __syncthreads();
 }
 
-// One wave per triplet.  Type 1: the block chi[i][j] depends on (i, j) only
-// through the monomial p = i + j, so H[p] = sum_pairs sum_l F_pair[p][l] Q[deg
-// p][l] with F_pair[p][l] = sum_m S_lm(k^_pair) poly2sph[p][lm] is formed once
-// per monomial (same terms, same order as type1.c:266-295 per element).
-__global__ __launch_bounds__(kAsmThreads) void k_assemble(DevTables T, DevJob J, AsmArgs A, unsigned char *status) {
+// Type-1 block of one triplet per wave (the triplets of one class
+// lambda = la + lb, listed in tlist).  chi[i][j] depends on (i, j) only
+// through the monomial p = i + j, so H[p] = sum_pairs sum_l F_pair[p][l]
+// Q[deg p][l] with F_pair[p][l] = sum_m S_lm(k^_pair) poly2sph[p][lm] is
+// formed once per monomial (same terms, same order as type1.c:266-295 per
+// element), then shifted to the output block (util.c:246-334).
+__global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, AsmArgs A,
+                                                      const unsigned char *__restrict__ status,
+                                                      const int *__restrict__ tlist,
+                                                      const unsigned char *__restrict__ fbflag, int filter) {
   extern __shared__ double lds[];
-  double *blk = lds + A.lds_blk;
   double *H = lds + A.lds_H;
   double *Jbuf = lds + A.lds_J;
   double *rsph = lds + A.lds_rsph;
-  double *Tl = lds + A.lds_T;
-  const int t = blockIdx.x;
+  const int t = tlist[blockIdx.x];
+  // filter 1: only triplets without large-grid work, 2: only those with it
+  if ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t])) return;
+  if (status[2 * t]) return;
   const ecp_triplet tr = J.trip[t];
   const ecp_centre Cn = J.cent[tr.centre];
   const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
@@ -1148,109 +1342,132 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(DevTables T, DevJob J,
   const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
   const double rAC[3] = {ra.x - rc.x, ra.y - rc.y, ra.z - rc.z};
   const double rBC[3] = {rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
+  const V3 vA{rAC[0], rAC[1], rAC[2]}, vB{rBC[0], rBC[1], rBC[2]};
   const int D = T.ijkDim;
   const int tmd = ECP_DIM_L(T.tmDim);
-  const bool fail1 = status[2 * t] != 0, fail2 = status[2 * t + 1] != 0;
+  (void)nca;
+  (void)ncb;
 
-  // ---- type 1 ----
-  if (!fail1) {
-    const int np = ECP_DIM_C(lab);
-    const int npairs = Sa.K * Sb.K;
-    const int u0 = A.t1first[t];
-    int qstart = 0;
-    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) H[idx] = 0.0;
-    for (int c0 = 0; c0 < npairs; c0 += A.pair_chunk) {
-      const int nc = min(A.pair_chunk, npairs - c0);
-      if ((int)threadIdx.x < nc) {
-        const T1Unit U = A.units1[u0 + c0 + threadIdx.x];
-        const double za = J.pa[Sa.prim + U.pa], zb = J.pa[Sb.prim + U.pb];
-        double P[3];
-        for (int p = 0; p < 3; p++) P[p] = 2.0 * (za * rAC[p] + zb * rBC[p]);
-        double r, th, ph;
-        sph_coords(P[0], P[1], P[2], r, th, ph);
-        real_sph_inplace(lab, th, ph, A.fac, A.dfac, rsph + threadIdx.x * A.lds_rs_stride);
-      }
-      __syncthreads();
-      for (int p = threadIdx.x; p < np; p += blockDim.x) {
-        const int *e = T.ijk + 3 * p;
-        const int deg = e[0] + e[1] + e[2];
-        qstart = 0;
-        for (int l1 = 0; l1 < deg; l1++) qstart += l1 / 2 + 1;
-        const double *pm = T.poly2sph + (size_t)p * tmd;
-        double acc = H[p];
-        for (int k = 0; k < nc; k++) {
-          const double *rs = rsph + k * A.lds_rs_stride;
-          const double *Q = A.qout + A.units1[u0 + c0 + k].qoff + qstart;
-          for (int l = deg; l >= 0; l -= 2) {
-            double f = 0.0;
-            for (int m = 0; m < 2 * l + 1; m++) f += rs[ECP_IDX_LM(l, m)] * pm[ECP_IDX_LM(l, m)];
-            acc += f * Q[(deg - l) / 2];
-          }
-        }
-        H[p] = acc;
-      }
-      __syncthreads();
+  const int np = ECP_DIM_C(lab);
+  const int npairs = Sa.K * Sb.K;
+  const int u0 = A.t1first[t];
+  int qstart = 0;
+  for (int idx = threadIdx.x; idx < np; idx += blockDim.x) H[idx] = 0.0;
+  for (int c0 = 0; c0 < npairs; c0 += A.pair_chunk) {
+    const int nc = min(A.pair_chunk, npairs - c0);
+    if ((int)threadIdx.x < nc) {
+      const T1Unit U = A.units1[u0 + c0 + threadIdx.x];
+      const double za = J.pa[Sa.prim + U.pa], zb = J.pa[Sb.prim + U.pb];
+      double P[3];
+      for (int p = 0; p < 3; p++) P[p] = 2.0 * (za * rAC[p] + zb * rBC[p]);
+      double r, th, ph;
+      sph_coords(P[0], P[1], P[2], r, th, ph);
+      real_sph_inplace(lab, th, ph, A.fac, A.dfac, rsph + threadIdx.x * A.lds_rs_stride);
     }
-    const double norm1 = 4.0 * M_PI;
-    auto get1 = [&](int r, int q) {
-      const int *ea = T.ijk + 3 * r, *eb = T.ijk + 3 * q;
-      return H[T.ijkIndex[((ea[0] + eb[0]) * D + ea[1] + eb[1]) * D + ea[2] + eb[2]]];
-    };
-    shift_block(T, get1, la, lb, rAC, rBC, norm1, Jbuf, A.blocks + tr.blk1);
-  }
-
-  // ---- type 2 ----
-  if (!fail2) {
-    for (int i = threadIdx.x; i < nca * ncb; i += blockDim.x) blk[i] = 0.0;
     __syncthreads();
-    const int nlm2 = J.angL * J.angL;
-    const double *angA = J.ang + Cn.ang_off + (size_t)(Sa.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
-    const double *angB = J.ang + Cn.ang_off + (size_t)(Sb.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
-    const int g0 = A.tgfirst[t];
-    for (int l = 0; l < Cn.L; l++) {
-      const T2Group G = A.groups[g0 + l];
-      const int d1 = la + l + 1, d2 = lb + l + 1, d3 = lab + 1;
-      for (int i = threadIdx.x; i < d1 * d2 * d3; i += blockDim.x) Tl[i] = 0.0;
-      __syncthreads();
-      for (int e = threadIdx.x; e < G.ne; e += blockDim.x) {
-        const int code = A.tmpl[G.tmpl + e];
-        const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
-        double v;
-        if (!A.tfail[G.eoff + e]) {
-          v = A.Tout[G.eoff + e];
-        } else {
-          v = 0.0;
-          const double *fb = A.fbT + (size_t)(G.eoff + e) * J.maxPairs;
-          for (int pr = 0; pr < Sa.K * Sb.K; pr++) v += fb[pr];
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+      const int *e = T.ijk + 3 * p;
+      const int deg = e[0] + e[1] + e[2];
+      qstart = 0;
+      for (int l1 = 0; l1 < deg; l1++) qstart += l1 / 2 + 1;
+      const double *pm = T.poly2sph + (size_t)p * tmd;
+      double acc = H[p];
+      for (int k = 0; k < nc; k++) {
+        const double *rs = rsph + k * A.lds_rs_stride;
+        const double *Q = A.qout + A.units1[u0 + c0 + k].qoff + qstart;
+        for (int l = deg; l >= 0; l -= 2) {
+          double f = 0.0;
+          for (int m = 0; m < 2 * l + 1; m++) f += rs[ECP_IDX_LM(l, m)] * pm[ECP_IDX_LM(l, m)];
+          acc += f * Q[(deg - l) / 2];
         }
-        Tl[(l1 * d2 + l2) * d3 + l3] = v;
       }
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < nca * ncb; idx += blockDim.x) {
-        const int pp = idx / ncb, qq = idx % ncb;
-        int alpha = 0, beta = 0;
-        while (ECP_DIM_C(alpha) <= pp) alpha++;
-        while (ECP_DIM_C(beta) <= qq) beta++;
-        const int par1 = (alpha + l) % 2, par2 = (beta + l) % 2;
-        const int ll1 = (par1 > l - alpha) ? par1 : l - alpha;
-        const int ll2 = (par2 > l - beta) ? par2 : l - beta;
-        double tmp = 0.0;
-        for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
-          for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
-            double factor = 0.0;
-            const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
-            const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
-            for (int m = 0; m < 2 * l + 1; m++) factor += pa[m * J.ang_c] * pb[m * J.ang_c];
-            tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
-          }
-        blk[idx] += tmp;
-      }
-      __syncthreads();
+      H[p] = acc;
     }
-    const double norm2 = (4.0 * M_PI) * (4.0 * M_PI);
-    auto get2 = [&](int r, int q) { return blk[r * ncb + q]; };
-    shift_block(T, get2, la, lb, rAC, rBC, norm2, Jbuf, A.blocks + tr.blk2);
+    __syncthreads();
   }
+  const double norm1 = 4.0 * M_PI;
+  auto get1 = [&](int r, int q) {
+    const int *ea = T.ijk + 3 * r, *eb = T.ijk + 3 * q;
+    return H[T.ijkIndex[((ea[0] + eb[0]) * D + ea[1] + eb[1]) * D + ea[2] + eb[2]]];
+  };
+  shift_block(T, get1, la, lb, vA, vB, norm1, Jbuf, A.blocks + tr.blk1);
+}
+
+// Type-2 block of one triplet per wave: for each projector l, the T_l
+// entries (small-grid results, or the sum of the per-pair fallback results)
+// are placed in LDS and contracted with the angular tables of both atoms
+// (type2.c:583-623), then the block is shifted (util.c:246-334).
+__global__ __launch_bounds__(kAsmThreads) void k_asm2(DevTables T, DevJob J, AsmArgs A,
+                                                      const unsigned char *__restrict__ status,
+                                                      const unsigned char *__restrict__ fbflag, int filter) {
+  extern __shared__ double lds[];
+  double *blk = lds + A.lds_blk;
+  double *Jbuf = lds + A.lds_J;
+  double *Tl = lds + A.lds_T;
+  const int t = blockIdx.x;
+  if ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t])) return;
+  if (status[2 * t + 1]) return;
+  const ecp_triplet tr = J.trip[t];
+  const ecp_centre Cn = J.cent[tr.centre];
+  const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
+  const int la = Sa.l, lb = Sb.l, lab = la + lb;
+  const int nca = ECP_DIM_C(la), ncb = ECP_DIM_C(lb);
+  const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
+  const double rAC[3] = {ra.x - rc.x, ra.y - rc.y, ra.z - rc.z};
+  const double rBC[3] = {rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
+  const V3 vA{rAC[0], rAC[1], rAC[2]}, vB{rBC[0], rBC[1], rBC[2]};
+  const int D = T.ijkDim;
+
+  for (int i = threadIdx.x; i < nca * ncb; i += blockDim.x) blk[i] = 0.0;
+  __syncthreads();
+  const int nlm2 = J.angL * J.angL;
+  const double *angA = J.ang + Cn.ang_off + (size_t)(Sa.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
+  const double *angB = J.ang + Cn.ang_off + (size_t)(Sb.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
+  const int g0 = A.tgfirst[t];
+  for (int l = 0; l < Cn.L; l++) {
+    const T2Group G = A.groups[g0 + l];
+    const int d1 = la + l + 1, d2 = lb + l + 1, d3 = lab + 1;
+    for (int i = threadIdx.x; i < d1 * d2 * d3; i += blockDim.x) Tl[i] = 0.0;
+    __syncthreads();
+    const int nm = 2 * l + 1;
+    for (int e = threadIdx.x; e < G.ne; e += blockDim.x) {
+      const int code = A.tmpl[G.tmpl + e];
+      const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
+      double v;
+      if (!A.tfail[G.eoff + e]) {
+        v = A.Tout[G.eoff + e];
+      } else {
+        v = 0.0;
+        const double *fb = A.fbT + (size_t)(G.eoff + e) * J.maxPairs;
+        for (int pr = 0; pr < Sa.K * Sb.K; pr++) v += fb[pr];
+      }
+      Tl[(l1 * d2 + l2) * d3 + l3] = v;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nca * ncb; idx += blockDim.x) {
+      const int pp = idx / ncb, qq = idx % ncb;
+      int alpha = 0, beta = 0;
+      while (ECP_DIM_C(alpha) <= pp) alpha++;
+      while (ECP_DIM_C(beta) <= qq) beta++;
+      const int par1 = (alpha + l) % 2, par2 = (beta + l) % 2;
+      const int ll1 = (par1 > l - alpha) ? par1 : l - alpha;
+      const int ll2 = (par2 > l - beta) ? par2 : l - beta;
+      double tmp = 0.0;
+      for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
+        for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
+          double factor = 0.0;
+          const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
+          const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
+          for (int m = 0; m < nm; m++) factor += pa[m * J.ang_c] * pb[m * J.ang_c];
+          tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
+        }
+      blk[idx] += tmp;
+    }
+    __syncthreads();
+  }
+  const double norm2 = (4.0 * M_PI) * (4.0 * M_PI);
+  auto get2 = [&](int r, int q) { return blk[r * ncb + q]; };
+  shift_block(T, get2, la, lb, vA, vB, norm2, Jbuf, A.blocks + tr.blk2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1275,8 +1492,8 @@ __global__ void k_zero(unsigned int *__restrict__ p, size_t nwords) {
 // first, then device-only buffers
 struct JobLayout {
   size_t i_atoms = 0, i_amaxl = 0, i_sh = 0, i_pa = 0, i_pd = 0, i_cent = 0, i_gauss = 0, i_win = 0, i_trip = 0,
-         i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, in_bytes = 0;
-  size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
+         i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, in_bytes = 0;
+  size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
          d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0;
 };
@@ -1300,6 +1517,9 @@ struct Prepared {
   unsigned long long *qf = nullptr;
   int *fbq1 = nullptr, *fbc = nullptr;
   unsigned char *stat = nullptr;
+  unsigned char *t1fb = nullptr, *t2fb = nullptr;  // per triplet: has large-grid work (type 1 / type 2)
+  ClassMap cm_head, cm_tail, cm_fb1, cm_fb2;      // fused launches
+  int max_lab = 0, max_lmb = 0;
   const T2Group *grp = nullptr;
   const int *goff = nullptr;
   double *Tout = nullptr;
@@ -1311,7 +1531,9 @@ struct Prepared {
   unsigned int *zero_base = nullptr;
   size_t zero_words = 0;
   double *blk = nullptr;
-  size_t asm_lds = 0;
+  size_t asm1_lds = 0, asm2_lds = 0;
+  int tcls_begin[ECP_MAX_LAB + 2] = {0};  // triplets of each class lambda = la + lb in tcls
+  const int *tcls = nullptr;
   // algorithmic FLOPs per evaluation (the per-unit models of the reference's
   // eager algorithm): type-1 radial work per class, type-2 radial work
   double flops_t1[ECP_MAX_LAB + 1] = {0};
@@ -1325,7 +1547,7 @@ constexpr int kWorkers = 4;
 struct ecp_dev {
   hipStream_t stream = nullptr;
   hipStream_t workers[kWorkers] = {};
-  hipEvent_t fork = nullptr, t2done = nullptr, join[kWorkers] = {};
+  hipEvent_t fork = nullptr, t2done = nullptr, t1done = nullptr, join[kWorkers] = {};
   DevTables T{};
   void *static_mem = nullptr;
   char *arena = nullptr;
@@ -1342,7 +1564,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 15, 100000, 8192};
+  ecp_tuning tune{8, 15, 100000, 8192, 2};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1447,6 +1669,7 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
   }
   if ((e = hipEventCreateWithFlags(&d->fork, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->t2done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
+  if ((e = hipEventCreateWithFlags(&d->t1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   for (int i = 0; i < 8; i++)
     if ((e = hipEventCreate(&d->ev[i])) != hipSuccess) return fail("hipEventCreate", e);
   build_templates(d);
@@ -1550,6 +1773,7 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   }
   if (d->fork) (void)hipEventDestroy(d->fork);
   if (d->t2done) (void)hipEventDestroy(d->t2done);
+  if (d->t1done) (void)hipEventDestroy(d->t1done);
   for (auto e : d->pev) (void)hipEventDestroy(e);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
@@ -1573,27 +1797,28 @@ namespace {
 // waves on the device) or the compacted lane-serial tail when it is large.
 template <int LAB>
 void launch_type1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T1Unit *u, int n, double *q,
-                  unsigned long long *qf, int *fbq, int *fbc, double *st, size_t plane, int *tq, int *tc) {
+                  unsigned long long *qf, int *fbq, int *fbc, double *st, size_t plane, int *tq, int *tc,
+                  unsigned char *ff) {
   if (n <= 0) return;
   const bool coop = n <= d->tune.t1_coop_max;
   const int head = coop ? d->tune.t1_head_coop : d->tune.t1_head_serial;
   const int grid = (n + 63) / 64;
   if (head >= T.s93.nlevels) {
     Span sp(d, s, FAM_TYPE1);
-    k_type1<LAB, 0><<<grid, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st, plane, tq, tc);
+    k_type1<LAB, 0><<<grid, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st, plane, tq, tc, ff);
     return;
   }
   {
     Span sp(d, s, FAM_TYPE1);
-    k_type1<LAB, 1><<<grid, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st, plane, tq, tc);
+    k_type1<LAB, 1><<<grid, 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st, plane, tq, tc, ff);
   }
   Span sp(d, s, FAM_T1TAIL);
   if (coop)
     k_t1tail<LAB><<<std::min(n, d->tune.t1_tail_grid), 64, 0, s>>>(T, J, u, q, qf, fbq, fbc, head, st, plane, tq,
-                                                                    tc);
+                                                                    tc, ff);
   else
     k_type1<LAB, 2><<<std::min(grid, d->tune.t1_tail_grid), 64, 0, s>>>(T, J, u, n, q, qf, fbq, fbc, head, st,
-                                                                         plane, tq, tc);
+                                                                         plane, tq, tc, ff);
 }
 
 template <int LAB>
@@ -1624,7 +1849,18 @@ void launch_fb2(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, 
 // type-2 kernel); the assembly joins them.  With evs, stage boundaries are
 // recorded on the main stream: start, tables done, quadratures done,
 // assembly done.
+void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs);
+
+// fuse = 2 (automatic): fused launches for jobs of up to this many type-1
+// primitive pairs (latency-bound: per-class launches hold few waves each)
+constexpr int kFuseAutoMaxUnits = 150000;
+
 void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
+  const int units = d->P.cls_begin[ECP_MAX_LAB + 1];
+  if (d->tune.fuse == 1 || (d->tune.fuse == 2 && units <= kFuseAutoMaxUnits)) {
+    enqueue_fused(d, s, evs);
+    return;
+  }
   const Prepared &P = d->P;
   const DevTables &T = d->T;
   const DevJob &J = P.J;
@@ -1644,7 +1880,8 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (P.ntot2 > 0) {
     Span sp(d, w[kWorkers - 1], FAM_TYPE2);
     k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w[kWorkers - 1]>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev,
-                                                                  P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups);
+                                                                  P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups,
+                                                                  P.t2fb);
   }
   (void)hipEventRecord(d->t2done, w[kWorkers - 1]);
   // type 1 classes, each followed by its fallback, round-robin over workers
@@ -1653,9 +1890,14 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     hipStream_t ws = w[(c) % kWorkers];                                                                         \
     launch_type1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                \
                     P.fbq1 + P.cls_begin[c], P.fbc + c, P.t1s, P.nq, P.tq1 + P.cls_begin[c],                     \
-                    P.fbc + 32 + c);                                                                            \
+                    P.fbc + 32 + c, P.t1fb);                                                                    \
     launch_fb1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                  \
                   P.fbq1 + P.cls_begin[c], P.fbc + c, P.stat);                                                  \
+    if (P.tcls_begin[(c) + 1] > P.tcls_begin[c]) {                                                            \
+      Span sp(d, ws, FAM_ASM);                                                                                  \
+      k_asm1<<<P.tcls_begin[(c) + 1] - P.tcls_begin[c], kAsmThreads, P.asm1_lds, ws>>>(                          \
+          T, J, P.aa, P.stat, P.tcls + P.tcls_begin[c], P.t1fb, 0);                                             \
+    }                                                                                                           \
   }
   T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10)
 #undef T1
@@ -1672,9 +1914,142 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (evs) (void)hipEventRecord(evs[2], s);
   {
     Span sp(d, s, FAM_ASM);
-    k_assemble<<<P.nt, kAsmThreads, P.asm_lds, s>>>(T, J, P.aa, P.stat);
+    k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, s>>>(T, J, P.aa, P.stat, P.t2fb, 0);
   }
   if (evs) (void)hipEventRecord(evs[3], s);
+}
+
+// workgroup ranges of the classes with work; blocks(c) = workgroups of class c
+template <typename F>
+ClassMap class_map(int ncls, const int *count, F blocks) {
+  ClassMap cm{};
+  int b = 0;
+  for (int c = 0; c < ncls; c++) {
+    if (count[c] <= 0) continue;
+    const int nb = blocks(c);
+    if (nb <= 0) continue;
+    cm.cls[cm.n] = c;
+    cm.bbeg[cm.n] = b;
+    b += nb;
+    cm.n++;
+  }
+  cm.bbeg[cm.n] = b;
+  return cm;
+}
+
+// The fused sequence: tables on the main stream, then two chains.  Type 1:
+// head (all classes) -> tail -> [assembly of triplets without large-grid
+// work] while, on a second stream, large-grid fallback -> assembly of the
+// rest.  Type 2 likewise: small grid -> [assembly without fallback] while
+// fallback -> assembly of the rest.  evs: stage boundaries as in enqueue.
+void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
+  const Prepared &P = d->P;
+  const DevTables &T = d->T;
+  const DevJob &J = P.J;
+  hipStream_t *w = d->workers;
+  if (evs) (void)hipEventRecord(evs[0], s);
+  {
+    Span sp(d, s, FAM_TABLES);
+    k_zero<<<(int)std::min<size_t>((P.zero_words + 255) / 256, 1024), 256, 0, s>>>(P.zero_base, P.zero_words);
+    k_utab<<<P.ncent, 384, 0, s>>>(T, J);
+    if (P.nfit) k_ftab<<<P.nfit, 128, 0, s>>>(T, J, P.fit);
+    if (P.nait) k_ang<<<P.nait, 128, 0, s>>>(T, J, P.ait, d->fac_dev, d->dfac_dev);
+  }
+  if (evs) (void)hipEventRecord(evs[1], s);
+  (void)hipEventRecord(d->fork, s);
+  for (int i = 0; i < kWorkers; i++) (void)hipStreamWaitEvent(w[i], d->fork, 0);
+  const ecp_tuning &tu = d->tune;
+  // ---- type 1 ----
+  T1Set S{};
+  S.units = P.u1;
+  S.q = P.q;
+  S.qf = P.qf;
+  S.fbq = P.fbq1;
+  S.fbc = P.fbc;
+  S.tq = P.tq1;
+  S.st = P.t1s;
+  S.plane = P.nq;
+  S.stat = P.stat;
+  S.fbflag = P.t1fb;
+  // cooperative tail for classes up to t1_coop_max pairs, none for larger
+  // ones (throughput-bound: the lane-serial head runs every level)
+  int tailcnt[ECP_MAX_LAB + 1];
+  bool tail = false;
+  for (int c = 0; c <= ECP_MAX_LAB; c++) {
+    S.ubeg[c] = P.cls_begin[c];
+    S.ucnt[c] = P.cls_count[c];
+    const bool coop = P.cls_count[c] <= tu.t1_coop_max && tu.t1_head_coop < T.s93.nlevels;
+    S.head[c] = coop ? tu.t1_head_coop : T.s93.nlevels;
+    tailcnt[c] = coop ? P.cls_count[c] : 0;
+    tail = tail || (coop && P.cls_count[c] > 0);
+  }
+  const bool big1 = P.max_lab > 6;
+  const ClassMap cmh = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return (P.cls_count[c] + 63) / 64; });
+  if (cmh.n > 0) {
+    {
+      Span sp(d, w[0], FAM_TYPE1);
+      if (big1) k_type1_fused<10, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
+      else k_type1_fused<6, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
+    }
+    if (tail) {
+      const ClassMap cmt = class_map(ECP_MAX_LAB + 1, tailcnt,
+                                     [&](int c) { return std::min(tailcnt[c], tu.t1_tail_grid); });
+      Span sp(d, w[0], FAM_T1TAIL);
+      if (big1) k_t1tail_fused<10><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
+      else k_t1tail_fused<6><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
+    }
+    (void)hipEventRecord(d->t1done, w[0]);
+    {
+      Span sp(d, w[0], FAM_ASM);
+      k_asm1<<<P.nt, kAsmThreads, P.asm1_lds, w[0]>>>(T, J, P.aa, P.stat, P.tcls, P.t1fb, 1);
+    }
+    (void)hipStreamWaitEvent(w[1], d->t1done, 0);
+    const ClassMap cmf = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return std::min(P.cls_count[c], 4096); });
+    {
+      Span sp(d, w[1], FAM_FB1);
+      if (big1) k_fb1_fused<10><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      else k_fb1_fused<6><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+    }
+    {
+      Span sp(d, w[1], FAM_ASM);
+      k_asm1<<<P.nt, kAsmThreads, P.asm1_lds, w[1]>>>(T, J, P.aa, P.stat, P.tcls, P.t1fb, 2);
+    }
+  }
+  // ---- type 2 ----
+  if (P.ntot2 > 0) {
+    {
+      Span sp(d, w[2], FAM_TYPE2);
+      k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w[2]>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout,
+                                                       P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups, P.t2fb);
+    }
+    (void)hipEventRecord(d->t2done, w[2]);
+    {
+      Span sp(d, w[2], FAM_ASM);
+      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, w[2]>>>(T, J, P.aa, P.stat, P.t2fb, 1);
+    }
+    (void)hipStreamWaitEvent(w[3], d->t2done, 0);
+    T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.ngroups};
+    const ClassMap cm2 = class_map(kMaxLMB + 1, P.lmb_count, [&](int c) {
+      return (int)std::min<long>((long)P.lmb_count[c] * P.maxPairs, 8192);
+    });
+    if (cm2.n > 0) {
+      Span sp(d, w[3], FAM_FB2);
+      if (P.max_lmb > 7) k_fb2_fused<11><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
+      else k_fb2_fused<7><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
+    }
+    {
+      Span sp(d, w[3], FAM_ASM);
+      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, w[3]>>>(T, J, P.aa, P.stat, P.t2fb, 2);
+    }
+  }
+  for (int i = 0; i < kWorkers; i++) {
+    (void)hipEventRecord(d->join[i], w[i]);
+    (void)hipStreamWaitEvent(s, d->join[i], 0);
+  }
+  if (evs) {
+    (void)hipEventRecord(evs[2], s);
+    (void)hipEventRecord(evs[3], s);
+  }
 }
 
 }  // namespace
@@ -1726,6 +2101,13 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     P.flops_t1[c] += (double)Sa.K * Sb.K * (G * (31.0 * c + 74 + 4.0 * nq_of(c)) + 150.0 * nq_of(c));
   }
   const size_t nq = cls_qbase[ECP_MAX_LAB + 1];
+  std::vector<int> tcls(nt);
+  {
+    for (int t = 0; t < nt; t++) P.tcls_begin[job->shells[job->trip[t].sa].l + job->shells[job->trip[t].sb].l + 1]++;
+    for (int c = 0; c <= ECP_MAX_LAB; c++) P.tcls_begin[c + 1] += P.tcls_begin[c];
+    std::vector<int> fill(P.tcls_begin, P.tcls_begin + ECP_MAX_LAB + 1);
+    for (int t = 0; t < nt; t++) tcls[fill[job->shells[job->trip[t].sa].l + job->shells[job->trip[t].sb].l]++] = t;
+  }
   std::vector<T2Group> groups;
   std::vector<int> goff, tgfirst(nt);
   int ntot2 = 0;
@@ -1746,6 +2128,10 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   }
   P.ngroups = (int)groups.size();
   P.ntot2 = ntot2;
+  for (int c = 0; c <= ECP_MAX_LAB; c++)
+    if (P.cls_count[c]) P.max_lab = c;
+  for (int c = 0; c <= kMaxLMB; c++)
+    if (P.lmb_count[c]) P.max_lmb = c;
   goff.push_back(ntot2);
   std::vector<int2> fitems, aitems;
   for (int c = 0; c < job->ncent; c++) {
@@ -1781,6 +2167,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.i_tgf = in_take(4 * (size_t)nt);
   Ly.i_fit = in_take(sizeof(int2) * fitems.size());
   Ly.i_ait = in_take(sizeof(int2) * aitems.size());
+  Ly.i_tcls = in_take(4 * (size_t)nt);
   Ly.in_bytes = in_off;
   size_t dv_off = align_up(Ly.in_bytes);
   auto dv_take = [&](size_t b) {
@@ -1792,6 +2179,8 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_fbc = dv_take(4 * 64);
   Ly.d_gflag = dv_take(4 * (size_t)P.ngroups);
   Ly.d_stat = dv_take(2 * (size_t)nt);
+  Ly.d_t1fb = dv_take((size_t)nt);
+  Ly.d_t2fb = dv_take((size_t)nt);
   Ly.d_zero_end = dv_off;
   Ly.d_UL = dv_take(8 * job->nUL);
   Ly.d_Ut = dv_take(8 * job->nUt);
@@ -1819,17 +2208,22 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     for (int i = 0; i < job->nsh; i++) LM = std::max(LM, job->shells[i].l);
     for (int c = 0; c < job->ncent; c++) LC = std::max(LC, job->cent[c].L);
     AsmArgs &aa = P.aa;
-    aa.pair_chunk = (LM <= 4) ? 16 : 8;
+    aa.pair_chunk = std::min((LM <= 4) ? 16 : 8, std::max(1, P.maxPairs));
     aa.lds_rs_stride = (2 * LM + 1) * (2 * LM + 1);
     int o = 0;
-    aa.lds_blk = o; o += ECP_DIM_C(LM) * ECP_DIM_C(LM);
-    aa.lds_H = o; o += ECP_DIM_C(2 * LM);
     aa.lds_J = o; o += ECP_DIM_IJK(LM) * ECP_DIM_C(LM);
+    const int base = o;
     aa.lds_rsph = o; o += aa.pair_chunk * aa.lds_rs_stride;
+    aa.lds_H = o; o += ECP_DIM_C(2 * LM);
+    const int end1 = o;
+    o = base;
+    aa.lds_blk = o; o += ECP_DIM_C(LM) * ECP_DIM_C(LM);
     aa.lds_T = o; o += (LM + LC) * (LM + LC) * (2 * LM + 1);
-    P.asm_lds = (size_t)o * sizeof(double);
-    if (P.asm_lds > 64 * 1024) {
-      snprintf(err, errlen, "assembly needs %zu bytes of LDS (l too high for this build)", P.asm_lds);
+    P.asm1_lds = (size_t)end1 * sizeof(double);
+    P.asm2_lds = (size_t)o * sizeof(double);
+    if (std::max(P.asm1_lds, P.asm2_lds) > 64 * 1024) {
+      snprintf(err, errlen, "assembly needs %zu bytes of LDS (l too high for this build)",
+               std::max(P.asm1_lds, P.asm2_lds));
       return -1;
     }
   }
@@ -1868,6 +2262,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   cp(Ly.i_tgf, tgfirst.data(), 4 * (size_t)nt);
   cp(Ly.i_fit, fitems.data(), sizeof(int2) * fitems.size());
   cp(Ly.i_ait, aitems.data(), sizeof(int2) * aitems.size());
+  cp(Ly.i_tcls, tcls.data(), 4 * (size_t)nt);
   if (bytes) *bytes = Ly.total;
   d->planned = true;
   return 0;
@@ -1935,6 +2330,8 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.fbq1 = (int *)(A + Ly.d_fbq1);
   P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 fallback queues, [16..27] type-2, [32..42] type-1 tails
   P.stat = (unsigned char *)(A + Ly.d_stat);
+  P.t1fb = (unsigned char *)(A + Ly.d_t1fb);
+  P.t2fb = (unsigned char *)(A + Ly.d_t2fb);
   P.grp = (const T2Group *)(A + Ly.i_grp);
   P.goff = (const int *)(A + Ly.i_goff);
   P.Tout = (double *)(A + Ly.d_T);
@@ -1944,6 +2341,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.fbT = (double *)(A + Ly.d_fbT);
   P.fit = (const int2 *)(A + Ly.i_fit);
   P.ait = (const int2 *)(A + Ly.i_ait);
+  P.tcls = (const int *)(A + Ly.i_tcls);
   P.blk = (double *)(A + Ly.d_blk);
   P.zero_base = (unsigned int *)(A + Ly.d_zero);
   P.zero_words = (Ly.d_zero_end - Ly.d_zero) / 4;

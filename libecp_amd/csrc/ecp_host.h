@@ -74,6 +74,9 @@ typedef struct {
   int t1_head_serial; /* ... before the lane-serial tail (>= the level count: no tail) */
   int t1_coop_max;    /* largest class (primitive pairs) that takes the cooperative tail */
   int t1_tail_grid;   /* most workgroups of a tail launch (persistent over its queue) */
+  int fuse;           /* 1: one launch per stage over all classes, assembly overlapping the
+                         large-grid fallbacks; 0: per-class launches on forked streams;
+                         2: fused for small jobs, per-class for large ones */
 } ecp_tuning;
 
 /* ---- device layer (ecp_device.hip) ---- */

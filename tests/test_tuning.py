@@ -9,12 +9,15 @@ from libecp_amd import native
 from libecp_amd.workloads import batch_molecule, medium_cluster
 
 SETTINGS = [
-    {},
-    {"t1_coop_max": 1 << 30, "t1_head_coop": 2},     # cooperative tail from level 2
-    {"t1_coop_max": 1 << 30, "t1_head_coop": 12},
-    {"t1_coop_max": 0, "t1_head_serial": 15},         # lane-serial only, no tail
-    {"t1_coop_max": 0, "t1_head_serial": 4},          # compacted lane-serial tail
-    {"t1_coop_max": 0, "t1_head_serial": 6, "t1_tail_grid": 3},
+    {"fuse": 0},
+    {"fuse": 1},                                                  # one launch per stage, all classes
+    {"fuse": 1, "t1_head_coop": 2, "t1_tail_grid": 5},
+    {"fuse": 1, "t1_head_coop": 15},                              # fused, no tail
+    {"fuse": 0, "t1_coop_max": 1 << 30, "t1_head_coop": 2},     # cooperative tail from level 2
+    {"fuse": 0, "t1_coop_max": 1 << 30, "t1_head_coop": 12},
+    {"fuse": 0, "t1_coop_max": 0, "t1_head_serial": 15},         # lane-serial only, no tail
+    {"fuse": 0, "t1_coop_max": 0, "t1_head_serial": 4},          # compacted lane-serial tail
+    {"fuse": 0, "t1_coop_max": 0, "t1_head_serial": 6, "t1_tail_grid": 3},
 ]
 
 
