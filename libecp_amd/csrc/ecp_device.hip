@@ -547,11 +547,6 @@ __global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Uni
 }
 
 // Wave-level helpers (blocks of exactly one 64-lane wave).
-__device__ __forceinline__ int wave_min(int v) {
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-  return v;
-}
-
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
 // Tail: one wave per primitive pair left open by the head, levels [head,
@@ -696,20 +691,58 @@ __device__ __forceinline__ Fm06 fm06(double zeta_p, double P) {
 // the numerical zero: the screening break of type1.c:222-225 /
 // type2.c:479-481; `order` if there is none.  expo(r) is evaluated by the
 // caller's functor with the same arithmetic as the integrand.
+// The predicate "r_n > max(dAC, dBC) and expo(r_n) < minExp" is monotone in
+// n (r_n increases with n, and beyond both centres the exponent decreases;
+// every rounding step is monotone), so its first true index is found with
+// two wave-wide probes instead of a scan: 64 samples at stride
+// ceil(order / 64) bracket it, then the 64 points ending at the first true
+// sample locate it.
 template <typename Expo>
 __device__ int fm06_break(const DevTables &T, const Fm06 &M, double dAC, double dBC, double minExp, Expo expo) {
   const int order = T.lorder;
-  for (int n0 = 0; n0 < order; n0 += 64) {
-    const int n = n0 + (int)threadIdx.x;
-    int cand = order;
-    if (n < order) {
-      const double r = M.i1 * T.xl[n] + M.i2;
-      if (r > dAC && r > dBC && expo(r) < minExp) cand = n;
+  auto pred = [&](int n) {
+    const double r = M.i1 * T.xl[n] + M.i2;
+    return r > dAC && r > dBC && expo(r) < minExp;
+  };
+  const int stride = (order + 63) / 64;
+  const int ns = min(order - 1, ((int)threadIdx.x + 1) * stride - 1);
+  const unsigned long long b1 = __ballot(pred(ns));
+  if (!b1) return order;
+  const int k0 = __ffsll((long long)b1) - 1;
+  const int hi = min(order - 1, (k0 + 1) * stride - 1);
+  const int n = hi - 63 + (int)threadIdx.x;
+  const unsigned long long b2 = __ballot(n >= 0 && pred(n));
+  return hi - 63 + (__ffsll((long long)b2) - 1);
+}
+
+// PSM92 evaluation rounds: the points of consecutive levels are evaluated by
+// the wave together when they fit (levels 0-1: 3 points, 2-5: 60, 6: 64);
+// larger levels go in chunks of 32 point pairs.  Within a round the centre
+// (level 0) takes slot 0 and pair k of level lv the slots roff + 2k (left)
+// and roff + 2k + 1 (right), roff being the points of the round's earlier
+// levels.  Summation order inside each level is unchanged.
+__device__ __forceinline__ int psm92_points(int lv) { return lv == 0 ? 1 : (1 << lv); }
+
+__device__ __forceinline__ int psm92_round_end(int lv, int nlev) {
+  return min(nlev, lv == 0 ? 2 : (lv == 2 ? 6 : lv + 1));
+}
+
+// grid index evaluated by `slot` in the round [l0, l1); -1 for none (a slot
+// past the round, or a right point past the screening end)
+__device__ __forceinline__ int psm92_round_point(const DevTables &T, int l0, int l1, int slot, int end) {
+  int o = 0;
+  for (int lv = l0; lv < l1; lv++) {
+    const int np = psm92_points(lv);
+    if (slot < o + np) {
+      if (lv == 0) return T.s92.centre;
+      const int pair = T.s92.lvl_begin[lv] + ((slot - o) >> 1);
+      if (((slot - o) & 1) == 0) return T.p92l[pair];
+      const int ri = T.p92r[pair];
+      return ri <= end ? ri : -1;
     }
-    cand = wave_min(cand);
-    if (cand < order) return cand;
+    o += np;
   }
-  return order;
+  return -1;
 }
 
 // type-1 fallback (type1.c:205-252): one wave per primitive pair with failed
@@ -787,38 +820,67 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
       return s_w[slot] * ((((Cc * rn) * s_u[slot]) * s_k[my_l2 * kKS + slot]) * s_g[slot]);
     };
     const bool mine = my_e >= 0;
-    if (lane == 0) eval(0, T.s92.centre);
-    __syncthreads();
-    double I = mine ? f(0) : 0.0, p = I, res = 0.0;
+    double I = 0.0, p = 0.0, res = 0.0;
     bool done = !mine;
-    __syncthreads();
-    for (int lv = 1; lv < T.s92.nlevels && wave_any(!done); lv++) {
-      const double q = 2 * p;
-      p = 2 * I;
-      const int b0 = T.s92.lvl_begin[lv], b1 = T.s92.lvl_begin[lv + 1];
-      for (int c0 = b0; c0 < b1; c0 += 32) {
-        const int nc = min(32, b1 - c0);
-        if (lane < nc) eval(lane, T.p92l[c0 + lane]);
-        else if (lane >= 32 && lane - 32 < nc) {
-          const int ri = T.p92r[c0 + lane - 32];
-          eval(lane, ri <= end ? ri : -1);
-        }
-        __syncthreads();
-        if (!done)
-          for (int k = 0; k < nc; k++) {
-            double t = f(k);
-            if (T.p92r[c0 + k] <= end) t += f(32 + k);
-            I += t;
-          }
-        __syncthreads();
+    auto test = [&](int l, double q) {
+      const double N = T.s92.lvl_n[l];
+      const double ee = I - p;
+      if (16 * ee * ee <= 3 * N * fabs(I - q) * T.tol) {
+        done = true;
+        res = 16 * I / (3 * N);
       }
-      const double N = T.s92.lvl_n[lv];
-      if (!done) {
-        const double ee = I - p;
-        if (16 * ee * ee <= 3 * N * fabs(I - q) * T.tol) {
-          done = true;
-          res = 16 * I / (3 * N);
+    };
+    // pairs [kb, ke) of level l, pair k at slot0 + 2 (k - kb)
+    auto sum = [&](int l, int slot0, int kb, int ke) {
+      const int b0 = T.s92.lvl_begin[l];
+      for (int k = kb; k < ke; k++) {
+        double t = f(slot0 + 2 * (k - kb));
+        if (T.p92r[b0 + k] <= end) t += f(slot0 + 2 * (k - kb) + 1);
+        I += t;
+      }
+    };
+    int lv = 0;
+    while (lv < T.s92.nlevels && wave_any(!done)) {
+      const int l1 = psm92_round_end(lv, T.s92.nlevels);
+      if (l1 > lv + 1 || psm92_points(lv) <= 64) {
+        eval(lane, psm92_round_point(T, lv, l1, lane, end));
+        __syncthreads();
+        int roff = 0;
+        for (int l = lv; l < l1; l++) {
+          if (l == 0) {
+            if (mine) I = p = f(0);
+          } else if (!done) {
+            const double q = 2 * p;
+            p = 2 * I;
+            sum(l, roff, 0, T.s92.lvl_begin[l + 1] - T.s92.lvl_begin[l]);
+            test(l, q);
+          }
+          roff += psm92_points(l);
         }
+        __syncthreads();
+        lv = l1;
+      } else {
+        double q = 0.0;
+        if (!done) {
+          q = 2 * p;
+          p = 2 * I;
+        }
+        const int b0 = T.s92.lvl_begin[lv], b1 = T.s92.lvl_begin[lv + 1];
+        for (int c0 = b0; c0 < b1; c0 += 32) {
+          const int nc = min(32, b1 - c0);
+          const int k = lane >> 1;
+          int n = -1;
+          if (k < nc) {
+            if ((lane & 1) == 0) n = T.p92l[c0 + k];
+            else if (T.p92r[c0 + k] <= end) n = T.p92r[c0 + k];
+          }
+          eval(lane, n);
+          __syncthreads();
+          if (!done) sum(lv, 0, c0 - b0, c0 - b0 + nc);
+          __syncthreads();
+        }
+        if (!done) test(lv, q);
+        lv++;
       }
     }
     if (mine) {
@@ -1038,40 +1100,8 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
         return s_w[slot] * ((((s_cu[slot] * rn) * s_ka[c1[j] * kKS + slot]) * s_kb[c2[j] * kKS + slot]) * s_ex[slot]);
       };
       __syncthreads();
-      if (lane == 0) eval(0, T.s92.centre);
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < kFb2PerLane; j++)
-        if (!(done & (1u << j))) I[j] = p[j] = f(j, 0);
-      __syncthreads();
-      for (int lv = 1; lv < T.s92.nlevels && wave_any(done != all); lv++) {
-        double q[kFb2PerLane];
-#pragma unroll
-        for (int j = 0; j < kFb2PerLane; j++) {
-          q[j] = 2 * p[j];
-          p[j] = 2 * I[j];
-        }
-        const int b0 = T.s92.lvl_begin[lv], b1 = T.s92.lvl_begin[lv + 1];
-        for (int cb = b0; cb < b1; cb += 32) {
-          const int nc = min(32, b1 - cb);
-          if (lane < nc) eval(lane, T.p92l[cb + lane]);
-          else if (lane >= 32 && lane - 32 < nc) {
-            const int ri = T.p92r[cb + lane - 32];
-            eval(lane, ri <= end ? ri : -1);
-          }
-          __syncthreads();
-#pragma unroll
-          for (int j = 0; j < kFb2PerLane; j++) {
-            if (done & (1u << j)) continue;
-            for (int k = 0; k < nc; k++) {
-              double t = f(j, k);
-              if (T.p92r[cb + k] <= end) t += f(j, 32 + k);
-              I[j] += t;
-            }
-          }
-          __syncthreads();
-        }
-        const double N = T.s92.lvl_n[lv];
+      auto test = [&](int l, const double (&q)[kFb2PerLane]) {
+        const double N = T.s92.lvl_n[l];
 #pragma unroll
         for (int j = 0; j < kFb2PerLane; j++) {
           if (done & (1u << j)) continue;
@@ -1080,6 +1110,69 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
             done |= 1u << j;
             res[j] = 16 * I[j] / (3 * N);
           }
+        }
+      };
+      auto sum = [&](int l, int slot0, int kb, int ke) {
+        const int b0 = T.s92.lvl_begin[l];
+#pragma unroll
+        for (int j = 0; j < kFb2PerLane; j++) {
+          if (done & (1u << j)) continue;
+          for (int k = kb; k < ke; k++) {
+            double t = f(j, slot0 + 2 * (k - kb));
+            if (T.p92r[b0 + k] <= end) t += f(j, slot0 + 2 * (k - kb) + 1);
+            I[j] += t;
+          }
+        }
+      };
+      int lv = 0;
+      while (lv < T.s92.nlevels && wave_any(done != all)) {
+        const int l1 = psm92_round_end(lv, T.s92.nlevels);
+        if (l1 > lv + 1 || psm92_points(lv) <= 64) {
+          eval(lane, psm92_round_point(T, lv, l1, lane, end));
+          __syncthreads();
+          int roff = 0;
+          for (int l = lv; l < l1; l++) {
+            if (l == 0) {
+#pragma unroll
+              for (int j = 0; j < kFb2PerLane; j++)
+                if (!(done & (1u << j))) I[j] = p[j] = f(j, 0);
+            } else {
+              double q[kFb2PerLane];
+#pragma unroll
+              for (int j = 0; j < kFb2PerLane; j++) {
+                q[j] = 2 * p[j];
+                p[j] = 2 * I[j];
+              }
+              sum(l, roff, 0, T.s92.lvl_begin[l + 1] - T.s92.lvl_begin[l]);
+              test(l, q);
+            }
+            roff += psm92_points(l);
+          }
+          __syncthreads();
+          lv = l1;
+        } else {
+          double q[kFb2PerLane];
+#pragma unroll
+          for (int j = 0; j < kFb2PerLane; j++) {
+            q[j] = 2 * p[j];
+            p[j] = 2 * I[j];
+          }
+          const int b0 = T.s92.lvl_begin[lv], b1 = T.s92.lvl_begin[lv + 1];
+          for (int cb = b0; cb < b1; cb += 32) {
+            const int nc = min(32, b1 - cb);
+            const int k = lane >> 1;
+            int n = -1;
+            if (k < nc) {
+              if ((lane & 1) == 0) n = T.p92l[cb + k];
+              else if (T.p92r[cb + k] <= end) n = T.p92r[cb + k];
+            }
+            eval(lane, n);
+            __syncthreads();
+            sum(lv, 0, cb - b0, cb - b0 + nc);
+            __syncthreads();
+          }
+          test(lv, q);
+          lv++;
         }
       }
 #pragma unroll
