@@ -922,7 +922,7 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Gr
                                                const int *__restrict__ tmpl, double *__restrict__ Tout,
                                                unsigned char *__restrict__ tfail, int *__restrict__ gflag,
                                                int *__restrict__ fbq, int *__restrict__ fbcount, int qstride,
-                                               unsigned char *__restrict__ fbflag) {
+                                               unsigned char *__restrict__ fbflag, int *__restrict__ flist) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= ntotal) return;
   const int g = find_group(goff, ngroups, gid);
@@ -973,10 +973,16 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Gr
   }
   Tout[gid] = res;
   tfail[gid] = ok ? 0 : 1;
-  if (!ok && atomicOr(&gflag[g], 1) == 0) {
-    const int slot = atomicAdd(&fbcount[G.lmb], 1);
-    fbq[(size_t)G.lmb * qstride + slot] = g;
-    fbflag[G.trip] = 1;
+  if (!ok) {
+    // gflag[g] counts the group's failed entries, listed (in any order) at
+    // flist[eoff ...]; the first one queues the group for the fallback
+    const int k = atomicAdd(&gflag[g], 1);
+    flist[G.eoff + k] = gid - G.eoff;
+    if (k == 0) {
+      const int slot = atomicAdd(&fbcount[G.lmb], 1);
+      fbq[(size_t)G.lmb * qstride + slot] = g;
+      fbflag[G.trip] = 1;
+    }
   }
 }
 
@@ -993,8 +999,9 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
                                          const T2Group *__restrict__ groups, const int *__restrict__ tmpl,
                                          const unsigned char *__restrict__ tfail, const int *__restrict__ fbq,
                                          const int *__restrict__ fbcount, double *__restrict__ fbT,
-                                         unsigned char *__restrict__ status, double *s_r, double *s_w, double *s_cu,
-                                         double *s_ex, double *s_ka, double *s_kb, int *s_ent) {
+                                         unsigned char *__restrict__ status, const int *__restrict__ gcount,
+                                         const int *__restrict__ flist, double *s_r, double *s_w, double *s_cu,
+                                         double *s_ex, double *s_ka, double *s_kb) {
   const int lane = threadIdx.x;
   const long total = (long)(*fbcount) * J.maxPairs;
   for (long it = bid; it < total; it += nb) {
@@ -1059,25 +1066,9 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
           if (l <= lbC) s_kb[l * kKS + slot] = K[l];
       }
     };
-    // failed entries of the group, in entry order, processed 256 per pass
-    int nfail = 0;
-    for (int e0 = 0; e0 < G.ne; e0 += 64) {
-      const bool fl = (e0 + lane < G.ne) && tfail[G.eoff + e0 + lane];
-      nfail += __popcll(__ballot(fl));
-    }
+    // the group's failed entries (listed by k_type2), 64 per pass
+    const int nfail = gcount[g];
     for (int pass0 = 0; pass0 < nfail; pass0 += 64 * kFb2PerLane) {
-      __syncthreads();
-      {
-        int base = 0;
-        for (int e0 = 0; e0 < G.ne; e0 += 64) {
-          const bool fl = (e0 + lane < G.ne) && tfail[G.eoff + e0 + lane];
-          const unsigned long long bal = __ballot(fl);
-          const int rank = base + __popcll(bal & ((1ull << lane) - 1ull));
-          if (fl && rank >= pass0 && rank < pass0 + 64 * kFb2PerLane) s_ent[rank - pass0] = e0 + lane;
-          base += __popcll(bal);
-        }
-      }
-      __syncthreads();
       const int nhere = min(64 * kFb2PerLane, nfail - pass0);
       int ent[kFb2PerLane], c1[kFb2PerLane], c2[kFb2PerLane], c3[kFb2PerLane];
       double I[kFb2PerLane], p[kFb2PerLane], res[kFb2PerLane];
@@ -1085,7 +1076,7 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
 #pragma unroll
       for (int j = 0; j < kFb2PerLane; j++) {
         const int k = lane + 64 * j;
-        ent[j] = (k < nhere) ? s_ent[k] : -1;
+        ent[j] = (k < nhere) ? flist[G.eoff + pass0 + k] : -1;
         const int code = ent[j] >= 0 ? tmpl[G.tmpl + ent[j]] : 0;
         c1[j] = code & 0xff;
         c2[j] = (code >> 8) & 0xff;
@@ -1190,12 +1181,12 @@ template <int LMB>
 __global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group *__restrict__ groups,
                                             const int *__restrict__ tmpl, const unsigned char *__restrict__ tfail,
                                             const int *__restrict__ fbq, const int *__restrict__ fbcount,
-                                            double *__restrict__ fbT, unsigned char *__restrict__ status) {
+                                            double *__restrict__ fbT, unsigned char *__restrict__ status,
+                                            const int *__restrict__ gcount, const int *__restrict__ flist) {
   __shared__ double s_r[64], s_w[64], s_cu[64], s_ex[64];
   __shared__ double s_ka[(LMB + 1) * kKS], s_kb[(LMB + 1) * kKS];
-  __shared__ int s_ent[64 * kFb2PerLane];
-  fb2_body<LMB>(blockIdx.x, gridDim.x, T, J, groups, tmpl, tfail, fbq, fbcount, fbT, status, s_r, s_w, s_cu, s_ex,
-                s_ka, s_kb, s_ent);
+  fb2_body<LMB>(blockIdx.x, gridDim.x, T, J, groups, tmpl, tfail, fbq, fbcount, fbT, status, gcount, flist, s_r, s_w,
+                s_cu, s_ex, s_ka, s_kb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1295,13 +1286,13 @@ struct T2Set {
   double *fbT;
   unsigned char *stat;
   int ngroups;
+  const int *gcount, *flist;
 };
 
 template <int MAXLMB>
 __global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S, ClassMap cm) {
   __shared__ double s_r[64], s_w[64], s_cu[64], s_ex[64];
   __shared__ double s_ka[(MAXLMB + 1) * kKS], s_kb[(MAXLMB + 1) * kKS];
-  __shared__ int s_ent[64 * kFb2PerLane];
   const int i = class_entry(cm, blockIdx.x);
   const int c = cm.cls[i], bid = blockIdx.x - cm.bbeg[i], nb = cm.bbeg[i + 1] - cm.bbeg[i];
   switch (c) {
@@ -1309,7 +1300,7 @@ __global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S
   case L:                                                                                                      \
     if constexpr (L <= MAXLMB)                                                                                 \
       fb2_body<L>(bid, nb, T, J, S.groups, S.tmpl, S.tf, S.fbq2 + (size_t)L * S.ngroups, S.fbc + 16 + L, S.fbT,   \
-                  S.stat, s_r, s_w, s_cu, s_ex, s_ka, s_kb, s_ent);                                             \
+                  S.stat, S.gcount, S.flist, s_r, s_w, s_cu, s_ex, s_ka, s_kb);                                 \
     break;
     ECP_CASES_LMB(X)
 #undef X
@@ -1587,7 +1578,7 @@ struct JobLayout {
   size_t i_atoms = 0, i_amaxl = 0, i_sh = 0, i_pa = 0, i_pd = 0, i_cent = 0, i_gauss = 0, i_win = 0, i_trip = 0,
          i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, in_bytes = 0;
   size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
-         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
+         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0;
 };
 
@@ -1618,6 +1609,7 @@ struct Prepared {
   double *Tout = nullptr;
   unsigned char *tf = nullptr;
   int *gflag = nullptr, *fbq2 = nullptr;
+  int *flist = nullptr;  // per group: its failed T entries (gflag[g] of them) at eoff
   double *fbT = nullptr;
   const int2 *fit = nullptr, *ait = nullptr;
   AsmArgs aa{};
@@ -1927,11 +1919,11 @@ void launch_fb1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, 
 template <int LMB>
 void launch_fb2(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T2Group *g, const int *tm,
                 const unsigned char *tf, const int *fbq, const int *fbc, double *fbT, unsigned char *st,
-                long maxItems) {
+                const int *gc, const int *fl, long maxItems) {
   if (maxItems > 0) {
     Span sp(d, s, FAM_FB2);
     const long blocks = std::min<long>(maxItems, 8192);
-    k_fb2<LMB><<<(int)blocks, 64, 0, s>>>(T, J, g, tm, tf, fbq, fbc, fbT, st);
+    k_fb2<LMB><<<(int)blocks, 64, 0, s>>>(T, J, g, tm, tf, fbq, fbc, fbT, st, gc, fl);
   }
 }
 
@@ -1974,7 +1966,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     Span sp(d, w[kWorkers - 1], FAM_TYPE2);
     k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w[kWorkers - 1]>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev,
                                                                   P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups,
-                                                                  P.t2fb);
+                                                                  P.t2fb, P.flist);
   }
   (void)hipEventRecord(d->t2done, w[kWorkers - 1]);
   // type 1 classes, each followed by its fallback, round-robin over workers
@@ -1997,7 +1989,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   for (int i = 0; i < kWorkers - 1; i++) (void)hipStreamWaitEvent(w[i], d->t2done, 0);
 #define FB2(c)                                                                                                  \
   launch_fb2<c>(d, w[(c) % kWorkers], T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.ngroups,            \
-                P.fbc + 16 + (c), P.fbT, P.stat, (long)P.lmb_count[c] * P.maxPairs);
+                P.fbc + 16 + (c), P.fbT, P.stat, P.gflag, P.flist, (long)P.lmb_count[c] * P.maxPairs);
   FB2(0) FB2(1) FB2(2) FB2(3) FB2(4) FB2(5) FB2(6) FB2(7) FB2(8) FB2(9) FB2(10) FB2(11)
 #undef FB2
   for (int i = 0; i < kWorkers; i++) {
@@ -2113,7 +2105,8 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     {
       Span sp(d, w[2], FAM_TYPE2);
       k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w[2]>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout,
-                                                       P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups, P.t2fb);
+                                                       P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups, P.t2fb,
+                                                       P.flist);
     }
     (void)hipEventRecord(d->t2done, w[2]);
     {
@@ -2121,7 +2114,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, w[2]>>>(T, J, P.aa, P.stat, P.t2fb, 1);
     }
     (void)hipStreamWaitEvent(w[3], d->t2done, 0);
-    T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.ngroups};
+    T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.ngroups, P.gflag, P.flist};
     const ClassMap cm2 = class_map(kMaxLMB + 1, P.lmb_count, [&](int c) {
       return (int)std::min<long>((long)P.lmb_count[c] * P.maxPairs, 8192);
     });
@@ -2285,6 +2278,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_fbq1 = dv_take(4 * units1.size());
   Ly.d_T = dv_take(8 * (size_t)ntot2);
   Ly.d_tf = dv_take((size_t)ntot2);
+  Ly.d_flist = dv_take(4 * (size_t)ntot2);
   Ly.d_fbq2 = dv_take(4 * (size_t)(kMaxLMB + 1) * P.ngroups);
   Ly.d_fbT = dv_take(8 * (size_t)ntot2 * P.maxPairs);
   Ly.d_blk = dv_take(8 * job->nblk);
@@ -2430,6 +2424,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.Tout = (double *)(A + Ly.d_T);
   P.tf = (unsigned char *)(A + Ly.d_tf);
   P.gflag = (int *)(A + Ly.d_gflag);
+  P.flist = (int *)(A + Ly.d_flist);
   P.fbq2 = (int *)(A + Ly.d_fbq2);
   P.fbT = (double *)(A + Ly.d_fbT);
   P.fit = (const int2 *)(A + Ly.i_fit);
