@@ -1927,13 +1927,14 @@ void launch_fb2(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, 
   }
 }
 
-// The kernel sequence of one evaluation, on resident inputs.  Tables run on
-// the main stream; the small-grid and fallback kernels of the different
-// classes are independent and run on kWorkers forked streams (each class's
-// fallback after its own small-grid kernel, the type-2 fallbacks after the
-// type-2 kernel); the assembly joins them.  With evs, stage boundaries are
-// recorded on the main stream: start, tables done, quadratures done,
-// assembly done.
+// The kernel sequence of one evaluation, on resident inputs, one launch per
+// class.  Tables run on the main stream.  The type-1 classes run on three
+// forked streams, each class as small grid -> large-grid fallback ->
+// assembly.  Type 2 runs on the fourth: small grid, then the assembly of the
+// triplets without type-2 fallback work, while the main stream runs the
+// fallbacks and the assembly of the rest.  With evs, stage boundaries are
+// recorded on the main stream: start, tables done, end, end (assembly
+// overlaps the quadratures).
 void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs);
 
 // fuse = 2 (automatic): fused launches for jobs of up to this many type-1
@@ -1961,18 +1962,37 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (evs) (void)hipEventRecord(evs[1], s);
   (void)hipEventRecord(d->fork, s);
   for (int i = 0; i < kWorkers; i++) (void)hipStreamWaitEvent(w[i], d->fork, 0);
-  // type 2 first on the last worker: its fallbacks wait for it
+  // type 2 on the last worker, then the assembly of the triplets without
+  // large-grid work there; the type-2 fallbacks and the assembly of the rest
+  // on the main stream, so that neither waits behind type-1 work
+  hipStream_t w2 = w[kWorkers - 1];
   if (P.ntot2 > 0) {
-    Span sp(d, w[kWorkers - 1], FAM_TYPE2);
-    k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w[kWorkers - 1]>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev,
-                                                                  P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups,
-                                                                  P.t2fb, P.flist);
+    {
+      Span sp(d, w2, FAM_TYPE2);
+      k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w2>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout,
+                                                     P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups, P.t2fb, P.flist);
+    }
+    (void)hipEventRecord(d->t2done, w2);
+    {
+      Span sp(d, w2, FAM_ASM);
+      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, w2>>>(T, J, P.aa, P.stat, P.t2fb, 1);
+    }
+    (void)hipStreamWaitEvent(s, d->t2done, 0);
+#define FB2(c)                                                                                                  \
+  launch_fb2<c>(d, s, T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.ngroups, P.fbc + 16 + (c), P.fbT,   \
+                P.stat, P.gflag, P.flist, (long)P.lmb_count[c] * P.maxPairs);
+    FB2(0) FB2(1) FB2(2) FB2(3) FB2(4) FB2(5) FB2(6) FB2(7) FB2(8) FB2(9) FB2(10) FB2(11)
+#undef FB2
+    {
+      Span sp(d, s, FAM_ASM);
+      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, s>>>(T, J, P.aa, P.stat, P.t2fb, 2);
+    }
   }
-  (void)hipEventRecord(d->t2done, w[kWorkers - 1]);
-  // type 1 classes, each followed by its fallback, round-robin over workers
+  // type 1 classes, each followed by its fallback and assembly, round-robin
+  // over the other workers
 #define T1(c)                                                                                                   \
   {                                                                                                             \
-    hipStream_t ws = w[(c) % kWorkers];                                                                         \
+    hipStream_t ws = w[(c) % (kWorkers - 1)];                                                                   \
     launch_type1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                \
                     P.fbq1 + P.cls_begin[c], P.fbc + c, P.t1s, P.nq, P.tq1 + P.cls_begin[c],                     \
                     P.fbc + 32 + c, P.t1fb);                                                                    \
@@ -1986,22 +2006,14 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   }
   T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10)
 #undef T1
-  for (int i = 0; i < kWorkers - 1; i++) (void)hipStreamWaitEvent(w[i], d->t2done, 0);
-#define FB2(c)                                                                                                  \
-  launch_fb2<c>(d, w[(c) % kWorkers], T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.ngroups,            \
-                P.fbc + 16 + (c), P.fbT, P.stat, P.gflag, P.flist, (long)P.lmb_count[c] * P.maxPairs);
-  FB2(0) FB2(1) FB2(2) FB2(3) FB2(4) FB2(5) FB2(6) FB2(7) FB2(8) FB2(9) FB2(10) FB2(11)
-#undef FB2
   for (int i = 0; i < kWorkers; i++) {
     (void)hipEventRecord(d->join[i], w[i]);
     (void)hipStreamWaitEvent(s, d->join[i], 0);
   }
-  if (evs) (void)hipEventRecord(evs[2], s);
-  {
-    Span sp(d, s, FAM_ASM);
-    k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, s>>>(T, J, P.aa, P.stat, P.t2fb, 0);
+  if (evs) {
+    (void)hipEventRecord(evs[2], s);
+    (void)hipEventRecord(evs[3], s);
   }
-  if (evs) (void)hipEventRecord(evs[3], s);
 }
 
 // workgroup ranges of the classes with work; blocks(c) = workgroups of class c
