@@ -72,8 +72,11 @@ const char *libecp_kernel_family(int i);
    "t1_tail_grid" (most workgroups of a tail launch), "fuse" (1: one launch
    per stage covering every class, with assembly overlapping the large-grid
    fallbacks; 0: per-class launches on forked streams; 2, the default: fused
-   for small jobs).  Results do not depend on them.  0, or -1 for an unknown
-   name. */
+   for small jobs), "asm2_cache" (type-2 assembly with the angular factors
+   in registers: 0 no, 1 yes, 2, the default: with fused launches),
+   "t2_head" (PS93 levels of the type-2 small-grid kernel before the entries
+   still open are rerun compacted; 15: no split), "t2_tail_grid".  Results do
+   not depend on them.  0, or -1 for an unknown name. */
 int libecp_batch_set_param(libecp_batch *b, const char *name, int value);
 int libecp_batch_get_param(libecp_batch *b, const char *name, int *value);
 /* Autotuner: times the batch (graph replays, `repeats` per candidate) over

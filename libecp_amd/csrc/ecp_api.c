@@ -396,6 +396,9 @@ static int *tuning_field(ecp_tuning *t, const char *name) {
   if (!strcmp(name, "t1_coop_max")) return &t->t1_coop_max;
   if (!strcmp(name, "t1_tail_grid")) return &t->t1_tail_grid;
   if (!strcmp(name, "fuse")) return &t->fuse;
+  if (!strcmp(name, "asm2_cache")) return &t->asm2_cache;
+  if (!strcmp(name, "t2_head")) return &t->t2_head;
+  if (!strcmp(name, "t2_tail_grid")) return &t->t2_tail_grid;
   return NULL;
 }
 
@@ -458,6 +461,8 @@ int libecp_batch_autotune(libecp_batch *b, int repeats, char *report, int len) {
     int v[6];
   } space[] = {
       {"fuse", 3, {0, 1, 2}},
+      {"asm2_cache", 3, {0, 1, 2}},
+      {"t2_head", 5, {4, 6, 8, 10, 15}},
       {"t1_coop_max", 5, {0, 20000, 100000, 400000, 1 << 30}},
       {"t1_head_coop", 4, {6, 8, 10, 12}},
       {"t1_head_serial", 4, {6, 8, 10, 15}},
@@ -486,8 +491,8 @@ int libecp_batch_autotune(libecp_batch *b, int repeats, char *report, int len) {
   }
   ecp_dev_set_tuning(b->dev, &best);
   if (report && used < (size_t)len)
-    snprintf(report + used, (size_t)len - used, "; best %.4f ms: fuse=%d t1_coop_max=%d t1_head_coop=%d "
-             "t1_head_serial=%d t1_tail_grid=%d", tbest, best.fuse, best.t1_coop_max, best.t1_head_coop,
-             best.t1_head_serial, best.t1_tail_grid);
+    snprintf(report + used, (size_t)len - used, "; best %.4f ms: fuse=%d asm2_cache=%d t2_head=%d t1_coop_max=%d "
+             "t1_head_coop=%d t1_head_serial=%d t1_tail_grid=%d", tbest, best.fuse, best.asm2_cache, best.t2_head,
+             best.t1_coop_max, best.t1_head_coop, best.t1_head_serial, best.t1_tail_grid);
   return 0;
 }

@@ -917,17 +917,11 @@ __device__ __forceinline__ int find_group(const int *__restrict__ goff, int ng, 
   return lo;
 }
 
-__global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Group *__restrict__ groups,
-                                               const int *__restrict__ goff, int ngroups, int ntotal,
-                                               const int *__restrict__ tmpl, double *__restrict__ Tout,
-                                               unsigned char *__restrict__ tfail, int *__restrict__ gflag,
-                                               int *__restrict__ fbq, int *__restrict__ fbcount, int qstride,
-                                               unsigned char *__restrict__ fbflag, int *__restrict__ flist) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= ntotal) return;
-  const int g = find_group(goff, ngroups, gid);
-  const T2Group G = groups[g];
-  const int code = tmpl[G.tmpl + (gid - G.eoff)];
+// One T entry: PS93 up to level lv_end (exclusive).  Returns 0 when it
+// converged (result in *res), 1 when every level ran without convergence,
+// 2 when it stopped unconverged at lv_end.
+__device__ __forceinline__ int t2_entry(const DevTables &T, const DevJob &J, const T2Group &G, int code, int lv_end,
+                                        double *res) {
   const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
   const ecp_triplet tr = J.trip[G.trip];
   const ecp_centre C = J.cent[tr.centre];
@@ -939,9 +933,8 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Gr
   double p = f(T.s93.centre);
   double q = f(T.s93.first_l) + f(T.s93.first_r);
   double I = p + q;
-  bool ok = false;
-  double res = 0.0;
-  for (int lv = 2; lv < T.s93.nlevels; lv++) {
+  *res = 0.0;
+  for (int lv = 2; lv < lv_end; lv++) {
     const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
     int cnt = 0;
     for (int k = T.s93.lvl_begin[lv]; k < T.s93.lvl_begin[lv + 1]; k++) {
@@ -966,23 +959,141 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, const T2Gr
       q = I;
     }
     if (cnt > 0 && err < T.tol) {
-      ok = true;
-      res = 16 * q / (3 * nn);
-      break;
+      *res = 16 * q / (3 * nn);
+      return 0;
     }
   }
-  Tout[gid] = res;
-  tfail[gid] = ok ? 0 : 1;
-  if (!ok) {
+  return lv_end < T.s93.nlevels ? 2 : 1;
+}
+
+struct T2Args {
+  const T2Group *groups;
+  const int *goff;
+  int ngroups, ntotal;
+  const int *tmpl;
+  double *Tout;
+  unsigned char *tfail;
+  int *gflag, *fbq, *fbcount;
+  int qstride;
+  unsigned char *fbflag;
+  int *flist;
+  int *tq, *tcount;  // entries left open by the head
+};
+
+__device__ __forceinline__ void t2_store(const T2Args &A, int gid, int g, const T2Group &G, int st, double res) {
+  A.Tout[gid] = res;
+  A.tfail[gid] = st == 0 ? 0 : 1;
+  if (st != 0) {
     // gflag[g] counts the group's failed entries, listed (in any order) at
     // flist[eoff ...]; the first one queues the group for the fallback
-    const int k = atomicAdd(&gflag[g], 1);
-    flist[G.eoff + k] = gid - G.eoff;
+    const int k = atomicAdd(&A.gflag[g], 1);
+    A.flist[G.eoff + k] = gid - G.eoff;
     if (k == 0) {
-      const int slot = atomicAdd(&fbcount[G.lmb], 1);
-      fbq[(size_t)G.lmb * qstride + slot] = g;
-      fbflag[G.trip] = 1;
+      const int slot = atomicAdd(&A.fbcount[G.lmb], 1);
+      A.fbq[(size_t)G.lmb * A.qstride + slot] = g;
+      A.fbflag[G.trip] = 1;
     }
+  }
+}
+
+// One lane per T entry.  With head < the level count, entries still open
+// after `head` levels are queued instead for k_type2_tail, so that the few
+// stragglers neither hold a wave of converged lanes nor run their remaining
+// levels one point at a time.
+__global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, T2Args A, int head) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= A.ntotal) return;
+  const int g = find_group(A.goff, A.ngroups, gid);
+  const T2Group G = A.groups[g];
+  const int code = A.tmpl[G.tmpl + (gid - G.eoff)];
+  double res;
+  const int st = t2_entry(T, J, G, code, min(head, T.s93.nlevels), &res);
+  if (st == 2) {
+    A.tq[atomicAdd(A.tcount, 1)] = gid;
+    return;
+  }
+  t2_store(A, gid, g, G, st, res);
+}
+
+// Tail: the entries left open by the head, rerun whole by kT2Group lanes
+// each (64 / kT2Group entries per wave).  Per level the group's lanes form
+// the point-pair sums T_k = (0 + f(left)) + f(right) into LDS and the
+// group's first lane adds them to I in pair order and applies the PS93 test,
+// so the arithmetic is that of the lane-serial integrator.
+constexpr int kT2Group = 4;
+
+__global__ __launch_bounds__(64) void k_type2_tail(DevTables T, DevJob J, T2Args A) {
+  constexpr int NE = 64 / kT2Group;
+  __shared__ double s_t[NE][64];
+  __shared__ int s_done[NE];
+  const int lane = threadIdx.x, e = lane / kT2Group, r = lane % kT2Group;
+  const int count = *A.tcount;
+  for (int it0 = blockIdx.x * NE; it0 < count; it0 += gridDim.x * NE) {
+    const int it = it0 + e;
+    const bool live = it < count;
+    int gid = 0, g = 0, code = 0;
+    T2Group G{};
+    if (live) {
+      gid = A.tq[it];
+      g = find_group(A.goff, A.ngroups, gid);
+      G = A.groups[g];
+      code = A.tmpl[G.tmpl + (gid - G.eoff)];
+    }
+    const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
+    const ecp_triplet tr = J.trip[G.trip];
+    const ecp_centre C = J.cent[tr.centre];
+    const double *__restrict__ Fa = J.F + C.F_off + ((size_t)(tr.sa - C.shell0) * J.FL + l1) * SO;
+    const double *__restrict__ Fb = J.F + C.F_off + ((size_t)(tr.sb - C.shell0) * J.FL + l2) * SO;
+    const double *__restrict__ Uu = J.Ut + C.Ut_off + ((size_t)G.l * J.UK + l3) * SO;
+    const int gs = tr.gs, ge = tr.ge;
+    auto f = [&](int n) { return T.ws[n] * ((Fa[n] * Fb[n]) * Uu[n]); };
+    double p = 0.0, q = 0.0, I = 0.0, res = 0.0;
+    bool done = !live;
+    if (live && r == 0) {
+      p = f(T.s93.centre);
+      q = f(T.s93.first_l) + f(T.s93.first_r);
+      I = p + q;
+    }
+    int st = 1;
+    for (int lv = 2; lv < T.s93.nlevels; lv++) {
+      if (!wave_any(!done)) break;
+      const int b0 = T.s93.lvl_begin[lv], np = T.s93.lvl_begin[lv + 1] - b0;
+      if (!done)
+        for (int k = r; k < np; k += kT2Group) {
+          const int li = T.p93l[b0 + k], ri = T.p93r[b0 + k];
+          double t = 0.0;
+          if (li >= gs) t += f(li);
+          if (ri <= ge) t += f(ri);
+          s_t[e][k] = t;
+        }
+      __syncthreads();
+      if (!done && r == 0) {
+        int cnt = 0;
+        for (int k = 0; k < np; k++) {
+          cnt += (T.p93l[b0 + k] >= gs) + (T.p93r[b0 + k] <= ge);
+          I += s_t[e][k];
+        }
+        const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
+        double err;
+        if (j == 0) {
+          p += (I - q);
+          err = 16 * fabs(q - 3 * p / 2) / (3 * nn);
+        } else {
+          err = 16 * fabs(I - 2 * q) / (3 * nn);
+          q = I;
+        }
+        if (cnt > 0 && err < T.tol) {
+          res = 16 * q / (3 * nn);
+          st = 0;
+          done = true;
+        }
+        s_done[e] = done;
+      }
+      __syncthreads();
+      done = s_done[e] || !live;
+      __syncthreads();
+    }
+    if (live && r == 0) t2_store(A, gid, g, G, st, res);
   }
 }
 
@@ -1481,6 +1592,10 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
 // entries (small-grid results, or the sum of the per-pair fallback results)
 // are placed in LDS and contracted with the angular tables of both atoms
 // (type2.c:583-623), then the block is shifted (util.c:246-334).
+// NB, NM > 0: the angular factors of atom B for every lambda2 (at most NB
+// values) and m (at most NM) are held in registers for each l, those of atom
+// A per lambda1; NB = 0 reads both from the table in the inner loop.
+template <int NB, int NM>
 __global__ __launch_bounds__(kAsmThreads) void k_asm2(DevTables T, DevJob J, AsmArgs A,
                                                       const unsigned char *__restrict__ status,
                                                       const unsigned char *__restrict__ fbflag, int filter) {
@@ -1537,14 +1652,42 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm2(DevTables T, DevJob J, Asm
       const int ll1 = (par1 > l - alpha) ? par1 : l - alpha;
       const int ll2 = (par2 > l - beta) ? par2 : l - beta;
       double tmp = 0.0;
-      for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
-        for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
-          double factor = 0.0;
-          const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
+      if constexpr (NB > 0) {
+        double Bv[NB][NM];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+          const int lam2 = ll2 + 2 * i;
           const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
-          for (int m = 0; m < nm; m++) factor += pa[m * J.ang_c] * pb[m * J.ang_c];
-          tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
+#pragma unroll
+          for (int m = 0; m < NM; m++) Bv[i][m] = (lam2 <= lb + l && m < nm) ? pb[m * J.ang_c] : 0.0;
         }
+        for (int lam1 = ll1; lam1 <= la + l; lam1 += 2) {
+          const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
+          double Av[NM];
+#pragma unroll
+          for (int m = 0; m < NM; m++) Av[m] = m < nm ? pa[m * J.ang_c] : 0.0;
+#pragma unroll
+          for (int i = 0; i < NB; i++) {
+            const int lam2 = ll2 + 2 * i;
+            if (lam2 <= lb + l) {
+              double factor = 0.0;
+#pragma unroll
+              for (int m = 0; m < NM; m++)
+                if (m < nm) factor += Av[m] * Bv[i][m];
+              tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
+            }
+          }
+        }
+      } else {
+        for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
+          for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
+            double factor = 0.0;
+            const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
+            const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
+            for (int m = 0; m < nm; m++) factor += pa[m * J.ang_c] * pb[m * J.ang_c];
+            tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
+          }
+      }
       blk[idx] += tmp;
     }
     __syncthreads();
@@ -1578,7 +1721,7 @@ struct JobLayout {
   size_t i_atoms = 0, i_amaxl = 0, i_sh = 0, i_pa = 0, i_pd = 0, i_cent = 0, i_gauss = 0, i_win = 0, i_trip = 0,
          i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, in_bytes = 0;
   size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
-         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
+         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0;
 };
 
@@ -1604,12 +1747,14 @@ struct Prepared {
   unsigned char *t1fb = nullptr, *t2fb = nullptr;  // per triplet: has large-grid work (type 1 / type 2)
   ClassMap cm_head, cm_tail, cm_fb1, cm_fb2;      // fused launches
   int max_lab = 0, max_lmb = 0;
+  int max_lbs = 0, max_L = 0;  // largest basis l and ECP local label of the job
   const T2Group *grp = nullptr;
   const int *goff = nullptr;
   double *Tout = nullptr;
   unsigned char *tf = nullptr;
   int *gflag = nullptr, *fbq2 = nullptr;
   int *flist = nullptr;  // per group: its failed T entries (gflag[g] of them) at eoff
+  int *tq2 = nullptr;    // type-2 entries left open by the head kernel
   double *fbT = nullptr;
   const int2 *fit = nullptr, *ait = nullptr;
   AsmArgs aa{};
@@ -1649,7 +1794,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 15, 100000, 8192, 2};
+  ecp_tuning tune{8, 15, 100000, 8192, 2, 2, 15, 2048};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1880,6 +2025,43 @@ namespace {
 // one lane-serial kernel; otherwise the head kernel, then for the pairs still
 // open the wave-cooperative tail when the class is small (latency-bound: few
 // waves on the device) or the compacted lane-serial tail when it is large.
+// type-2 small grid: the lane-per-entry kernel, split into head and
+// compacted tail when t2_head is below the level count
+void launch_type2(ecp_dev *d, hipStream_t s) {
+  const Prepared &P = d->P;
+  if (P.ntot2 <= 0) return;
+  T2Args A{P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups,
+           P.t2fb, P.flist, P.tq2, P.fbc + 48};
+  const int head = d->tune.t2_head;
+  {
+    Span sp(d, s, FAM_TYPE2);
+    k_type2<<<(P.ntot2 + 127) / 128, 128, 0, s>>>(d->T, P.J, A, head);
+  }
+  if (head < d->T.s93.nlevels) {
+    Span sp(d, s, FAM_TYPE2);
+    k_type2_tail<<<std::min((P.ntot2 + 15) / 16, d->tune.t2_tail_grid), 64, 0, s>>>(d->T, P.J, A);
+  }
+}
+
+// type-2 assembly: the register-cached variant fits basis l <= 3 with
+// projectors l <= 3 (NB = 4 lambda2 values, NM = 7 m values) or basis
+// l <= 4 with projectors l <= 4; larger jobs read the tables directly.  The
+// cache costs occupancy: it pays where the assembly is on the critical path
+// of a small job (fused launches), not in large throughput-bound ones.
+void launch_asm2(ecp_dev *d, hipStream_t s, int filter, bool fused) {
+  const Prepared &P = d->P;
+  const int mode = d->tune.asm2_cache;
+  const bool cache = mode == 1 || (mode == 2 && fused);
+  if (!cache)
+    k_asm2<0, 1><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  else if (P.max_lbs <= 3 && P.max_L <= 4)
+    k_asm2<4, 7><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  else if (P.max_lbs <= 4 && P.max_L <= 5)
+    k_asm2<5, 9><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  else
+    k_asm2<0, 1><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+}
+
 template <int LAB>
 void launch_type1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T1Unit *u, int n, double *q,
                   unsigned long long *qf, int *fbq, int *fbc, double *st, size_t plane, int *tq, int *tc,
@@ -1967,15 +2149,11 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   // on the main stream, so that neither waits behind type-1 work
   hipStream_t w2 = w[kWorkers - 1];
   if (P.ntot2 > 0) {
-    {
-      Span sp(d, w2, FAM_TYPE2);
-      k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w2>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout,
-                                                     P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups, P.t2fb, P.flist);
-    }
+    launch_type2(d, w2);
     (void)hipEventRecord(d->t2done, w2);
     {
       Span sp(d, w2, FAM_ASM);
-      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, w2>>>(T, J, P.aa, P.stat, P.t2fb, 1);
+      launch_asm2(d, w2, 1, false);
     }
     (void)hipStreamWaitEvent(s, d->t2done, 0);
 #define FB2(c)                                                                                                  \
@@ -1985,7 +2163,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
 #undef FB2
     {
       Span sp(d, s, FAM_ASM);
-      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, s>>>(T, J, P.aa, P.stat, P.t2fb, 2);
+      launch_asm2(d, s, 2, false);
     }
   }
   // type 1 classes, each followed by its fallback and assembly, round-robin
@@ -2114,16 +2292,11 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   }
   // ---- type 2 ----
   if (P.ntot2 > 0) {
-    {
-      Span sp(d, w[2], FAM_TYPE2);
-      k_type2<<<(P.ntot2 + 127) / 128, 128, 0, w[2]>>>(T, J, P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout,
-                                                       P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups, P.t2fb,
-                                                       P.flist);
-    }
+    launch_type2(d, w[2]);
     (void)hipEventRecord(d->t2done, w[2]);
     {
       Span sp(d, w[2], FAM_ASM);
-      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, w[2]>>>(T, J, P.aa, P.stat, P.t2fb, 1);
+      launch_asm2(d, w[2], 1, true);
     }
     (void)hipStreamWaitEvent(w[3], d->t2done, 0);
     T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.ngroups, P.gflag, P.flist};
@@ -2137,7 +2310,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     }
     {
       Span sp(d, w[3], FAM_ASM);
-      k_asm2<<<P.nt, kAsmThreads, P.asm2_lds, w[3]>>>(T, J, P.aa, P.stat, P.t2fb, 2);
+      launch_asm2(d, w[3], 2, true);
     }
   }
   for (int i = 0; i < kWorkers; i++) {
@@ -2291,6 +2464,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_T = dv_take(8 * (size_t)ntot2);
   Ly.d_tf = dv_take((size_t)ntot2);
   Ly.d_flist = dv_take(4 * (size_t)ntot2);
+  Ly.d_tq2 = dv_take(4 * (size_t)ntot2);
   Ly.d_fbq2 = dv_take(4 * (size_t)(kMaxLMB + 1) * P.ngroups);
   Ly.d_fbT = dv_take(8 * (size_t)ntot2 * P.maxPairs);
   Ly.d_blk = dv_take(8 * job->nblk);
@@ -2306,6 +2480,8 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     int LM = 0, LC = 1;
     for (int i = 0; i < job->nsh; i++) LM = std::max(LM, job->shells[i].l);
     for (int c = 0; c < job->ncent; c++) LC = std::max(LC, job->cent[c].L);
+    P.max_lbs = LM;
+    P.max_L = LC;
     AsmArgs &aa = P.aa;
     aa.pair_chunk = std::min((LM <= 4) ? 16 : 8, std::max(1, P.maxPairs));
     aa.lds_rs_stride = (2 * LM + 1) * (2 * LM + 1);
@@ -2427,7 +2603,8 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.tq1 = (int *)(A + Ly.d_tq1);
   P.qf = (unsigned long long *)(A + Ly.d_qf);
   P.fbq1 = (int *)(A + Ly.d_fbq1);
-  P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 fallback queues, [16..27] type-2, [32..42] type-1 tails
+  P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 fallback queues, [16..27] type-2, [32..42] type-1 tails,
+                                  // [48] type-2 tail
   P.stat = (unsigned char *)(A + Ly.d_stat);
   P.t1fb = (unsigned char *)(A + Ly.d_t1fb);
   P.t2fb = (unsigned char *)(A + Ly.d_t2fb);
@@ -2437,6 +2614,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.tf = (unsigned char *)(A + Ly.d_tf);
   P.gflag = (int *)(A + Ly.d_gflag);
   P.flist = (int *)(A + Ly.d_flist);
+  P.tq2 = (int *)(A + Ly.d_tq2);
   P.fbq2 = (int *)(A + Ly.d_fbq2);
   P.fbT = (double *)(A + Ly.d_fbT);
   P.fit = (const int2 *)(A + Ly.i_fit);

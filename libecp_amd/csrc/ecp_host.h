@@ -77,6 +77,11 @@ typedef struct {
   int fuse;           /* 1: one launch per stage over all classes, assembly overlapping the
                          large-grid fallbacks; 0: per-class launches on forked streams;
                          2: fused for small jobs, per-class for large ones */
+  int asm2_cache;     /* type-2 assembly with angular factors in registers: 0 no, 1 yes,
+                         2 with fused launches only */
+  int t2_head;        /* PS93 levels of the type-2 head kernel before the compacted tail
+                         (>= the level count: no tail) */
+  int t2_tail_grid;   /* most workgroups of the type-2 tail launch */
 } ecp_tuning;
 
 /* ---- device layer (ecp_device.hip) ---- */
