@@ -20,8 +20,9 @@ extern "C" {
 
 typedef struct _libECPHandle libECPHandle;
 
-/* Receives one block per (C, shell a, shell b >= a, type): an
-   IJK(la+shifta) x IJK(lb+shiftb) row-major array of Cartesian integrals in
+/* Receives one block per (C, shell a, shell b >= a, shift, type): an
+   IJK(la+shifta) x IJK(lb+shiftb) row-major array (for n = 2 the (1,0) and
+   (0,1) shifts keep the unshifted l: IJK(la) x IJK(lb)) of Cartesian integrals in
    libint order (p: x,y,z; d: xx,xy,xz,yy,yz,zz; ...).  The library owns the
    block; it is valid only during the call.  Calls arrive on the caller's
    thread in the reference order C -> A -> B >= A -> sa -> sb -> shift ->
@@ -35,9 +36,11 @@ typedef void (*ECPCallback)(int A, int sa, int la, int shifta,
    primitive dBS/aBS (used unnormalised).  ECP: shellsECP[atom] (0 = none),
    per ECP shell lECP/KECP, per primitive nECP (radial power including the
    r^2 Jacobian), dECP, aECP; the highest l of an atom is its local channel.
-   n: derivative order (this build: 0).  lmax/shellOrdering: custom Cartesian
-   ordering (NULL = libint).  Returns NULL on invalid input or when the
-   Bessel tabulation does not converge. */
+   n: derivative order 0, 1 or 2 (basis l + n at most 5); for n > 0 each
+   triplet is delivered once per angular-momentum shift of the reference's
+   table (libecp.c:243-250), with the shifts in the callback.
+   lmax/shellOrdering: custom Cartesian ordering (NULL = libint).  Returns
+   NULL on invalid input or when the Bessel tabulation does not converge. */
 libECPHandle *libECP_init(int nrAtoms, double *geometry,
                           int *shellsECP, int *lECP, int *KECP,
                           double *nECP, double *dECP, double *aECP,

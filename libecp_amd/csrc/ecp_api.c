@@ -73,8 +73,8 @@ libECPHandle *libECP_init(int nrAtoms, double *geometry, int *shellsECP, int *lE
                           double accuracy) {
   char err[512] = "";
   set_last_error("");
-  if (n != 0) {
-    set_last_error("derivative integrals (n > 0) are not available in this build");
+  if (n < 0 || n > 2) {
+    set_last_error("derivative order n must be 0, 1 or 2");
     return NULL;
   }
   if (nrAtoms <= 0 || !geometry || !shellsECP || !shellsBS) {
@@ -144,7 +144,7 @@ static int run_handle(libECPHandle *h) {
     ecp_job_free(&h->job);
     h->job_ready = 0;
   }
-  if (ecp_job_build(&h->mol, 1, &h->st, &h->job, err, sizeof(err))) {
+  if (ecp_job_build(&h->mol, 1, h->n, &h->st, &h->job, err, sizeof(err))) {
     set_last_error(err);
     return -1;
   }
@@ -183,9 +183,9 @@ int calculateECPIntegrals(libECPHandle *h, ECPCallback cb, void *args) {
     const int A = Sa->atom, B = Sb->atom, C = J->cent[tr->centre].atom;
     const int sa = tr->sa - J->atom_shell0[A], sb = tr->sb - J->atom_shell0[B];
     if (h->status[2 * t]) return 1;
-    if (cb) cb(A, sa, Sa->l, 0, B, sb, Sb->l, 0, C, h->blocks + tr->blk1, args);
+    if (cb) cb(A, sa, Sa->l, tr->shifta, B, sb, Sb->l, tr->shiftb, C, h->blocks + tr->blk1, args);
     if (h->status[2 * t + 1]) return 2;
-    if (cb) cb(A, sa, Sa->l, 0, B, sb, Sb->l, 0, C, h->blocks + tr->blk2, args);
+    if (cb) cb(A, sa, Sa->l, tr->shifta, B, sb, Sb->l, tr->shiftb, C, h->blocks + tr->blk2, args);
   }
   return 0;
 }
@@ -228,7 +228,7 @@ int libecp_screened_triplets(libECPHandle *h, int *keys, int cap) {
     ecp_job_free(&h->job);
     h->job_ready = 0;
   }
-  if (ecp_job_build(&h->mol, 1, &h->st, &h->job, err, sizeof(err))) {
+  if (ecp_job_build(&h->mol, 1, h->n, &h->st, &h->job, err, sizeof(err))) {
     set_last_error(err);
     return -1;
   }
@@ -313,7 +313,7 @@ libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int lar
     if (d.maxLBS > maxLBS) maxLBS = d.maxLBS;
   }
   if (ecp_static_init(&b->st, maxL, maxLBS, NULL, -1, largeGridOrder, tolerance, accuracy, err, sizeof(err)) ||
-      ecp_job_build(b->mols, nmol, &b->st, &b->job, err, sizeof(err))) {
+      ecp_job_build(b->mols, nmol, 0, &b->st, &b->job, err, sizeof(err))) {
     set_last_error(err);
     libecp_batch_free(b);
     return NULL;

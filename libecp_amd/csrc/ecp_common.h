@@ -94,6 +94,11 @@ typedef struct {
   int sa, sb;      /* global shell indices */
   int gs, ge;      /* pair window: [max start, max end] */
   int blk1, blk2;  /* output offsets (doubles) of the type-1 / type-2 blocks */
+  int la, lb;      /* angular momenta of the integrals: shell l + shift (derivatives) */
+  int oa, ob;      /* angular momenta of the output block (la, lb, except the unshifted
+                      side of the second-derivative shifts (+1, 0) and (0, +1)) */
+  int powa, powb;  /* contraction coefficients are scaled by zeta^pow (positive shifts) */
+  int shifta, shiftb;
 } ecp_triplet;
 
 #endif

@@ -83,6 +83,7 @@ struct DevJob {
   double *UL, *Ut, *F, *ang;
   int FL, UK, angL, ang_lam, ang_c;
   int maxPairs;
+  int deriv;  // derivative order: F slabs per run 0..deriv, angular tables to lmax + deriv
 };
 
 // ---------------------------------------------------------------------------
@@ -269,13 +270,15 @@ __global__ __launch_bounds__(384) void k_utab(DevTables T, DevJob J) {
   }
 }
 
-// F[centre][shell][lambda][n] (type2.c:263-303), one block per non-skipped
-// (centre, shell); the whole slab is written (zeros outside the window).
+// F[centre][run][shell][lambda][n] (type2.c:263-303), one block per
+// non-skipped (centre, shell, run); the whole slab is written (zeros outside
+// the window).
 template <int LM>
-__device__ void ftab_slab(const DevTables &T, const DevJob &J, const ecp_centre &C, int sl, const ecp_window &W) {
+__device__ void ftab_slab(const DevTables &T, const DevJob &J, const ecp_centre &C, int sl, int run,
+                          const ecp_window &W) {
   const ecp_shell S = J.shells[C.shell0 + sl];
   const double dAC = dist(J.atoms[C.atom], J.atoms[S.atom]);
-  double *slab = J.F + C.F_off + (size_t)sl * J.FL * SO;
+  double *slab = J.F + C.F_off + ((size_t)run * C.nsh + sl) * J.FL * SO;
   for (int n = threadIdx.x; n < SO; n += blockDim.x) {
     double acc[LM + 1];
 #pragma unroll
@@ -283,7 +286,10 @@ __device__ void ftab_slab(const DevTables &T, const DevJob &J, const ecp_centre 
     if (n >= W.start && n < W.end) {
       const double r = T.xs[n];
       for (int p = 0; p < S.K; p++) {
-        const double zeta = J.pa[S.prim + p], da = J.pd[S.prim + p];
+        const double zeta = J.pa[S.prim + p];
+        double da = J.pd[S.prim + p];
+        // derivatives: run n carries a^n (type2.c:286-289)
+        for (int k = 0; k < run; k++) da *= zeta;
         double K[LM + 1];
         bessel_k<LM>(T.bessel, 2.0 * zeta * dAC * r, K);
         double e = dAC - r;
@@ -300,13 +306,14 @@ __device__ void ftab_slab(const DevTables &T, const DevJob &J, const ecp_centre 
 
 __global__ __launch_bounds__(128) void k_ftab(DevTables T, DevJob J, const int2 *__restrict__ items) {
   const int2 it = items[blockIdx.x];
+  const int sl = it.y >> 2, run = it.y & 3;  // (shell, run) packed
   const ecp_centre C = J.cent[it.x];
-  const ecp_window W = J.win[C.win_off + it.y];
-  const int lm = C.L - 1 + J.shells[C.shell0 + it.y].l;
+  const ecp_window W = J.win[C.win_off + sl];
+  const int lm = C.L - 1 + J.shells[C.shell0 + sl].l + run;
   switch (lm) {
 #define FT(k) \
   case k:     \
-    ftab_slab<k>(T, J, C, it.y, W); \
+    ftab_slab<k>(T, J, C, sl, run, W); \
     break;
     FT(0) FT(1) FT(2) FT(3) FT(4) FT(5) FT(6) FT(7) FT(8) FT(9) FT(10) FT(11)
 #undef FT
@@ -322,7 +329,7 @@ __global__ __launch_bounds__(128) void k_ang(DevTables T, DevJob J, const int2 *
   const int2 it = items[blockIdx.x];
   const ecp_centre C = J.cent[it.x];
   const int A = C.atom0 + it.y;
-  const int lmax_a = J.atom_maxl[A];
+  const int lmax_a = J.atom_maxl[A] + J.deriv;
   const int lmax = C.L - 1 + lmax_a;
   if (threadIdx.x == 0) {
     const ecp_xyz a = J.atoms[A], c = J.atoms[C.atom];
@@ -376,6 +383,9 @@ __device__ __forceinline__ void t1_geom(const DevJob &J, const ecp_triplet &tr, 
   G.ca = J.pd[Sa.prim + pa];
   G.zb = J.pa[Sb.prim + pb];
   G.cb = J.pd[Sb.prim + pb];
+  // derivatives: coefficients times zeta^shift (type1.c:239-246)
+  for (int k = 0; k < tr.powa; k++) G.ca *= G.za;
+  for (int k = 0; k < tr.powb; k++) G.cb *= G.zb;
   for (int p = 0; p < 3; p++) G.P[p] = 2.0 * (G.za * rAC[p] + G.zb * rBC[p]);
   G.s = sqrt(G.P[0] * G.P[0] + G.P[1] * G.P[1] + G.P[2] * G.P[2]);
   G.zd2 = -G.za * G.dAC * G.dAC - G.zb * G.dBC * G.dBC;
@@ -925,8 +935,10 @@ __device__ __forceinline__ int t2_entry(const DevTables &T, const DevJob &J, con
   const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
   const ecp_triplet tr = J.trip[G.trip];
   const ecp_centre C = J.cent[tr.centre];
-  const double *__restrict__ Fa = J.F + C.F_off + ((size_t)(tr.sa - C.shell0) * J.FL + l1) * SO;
-  const double *__restrict__ Fb = J.F + C.F_off + ((size_t)(tr.sb - C.shell0) * J.FL + l2) * SO;
+  const double *__restrict__ Fa =
+      J.F + C.F_off + (((size_t)tr.powa * C.nsh + (tr.sa - C.shell0)) * J.FL + l1) * SO;
+  const double *__restrict__ Fb =
+      J.F + C.F_off + (((size_t)tr.powb * C.nsh + (tr.sb - C.shell0)) * J.FL + l2) * SO;
   const double *__restrict__ Uu = J.Ut + C.Ut_off + ((size_t)G.l * J.UK + l3) * SO;
   const int gs = tr.gs, ge = tr.ge;
   auto f = [&](int n) { return T.ws[n] * ((Fa[n] * Fb[n]) * Uu[n]); };
@@ -1042,8 +1054,10 @@ __global__ __launch_bounds__(64) void k_type2_tail(DevTables T, DevJob J, T2Args
     const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
     const ecp_triplet tr = J.trip[G.trip];
     const ecp_centre C = J.cent[tr.centre];
-    const double *__restrict__ Fa = J.F + C.F_off + ((size_t)(tr.sa - C.shell0) * J.FL + l1) * SO;
-    const double *__restrict__ Fb = J.F + C.F_off + ((size_t)(tr.sb - C.shell0) * J.FL + l2) * SO;
+    const double *__restrict__ Fa =
+        J.F + C.F_off + (((size_t)tr.powa * C.nsh + (tr.sa - C.shell0)) * J.FL + l1) * SO;
+    const double *__restrict__ Fb =
+        J.F + C.F_off + (((size_t)tr.powb * C.nsh + (tr.sb - C.shell0)) * J.FL + l2) * SO;
     const double *__restrict__ Uu = J.Ut + C.Ut_off + ((size_t)G.l * J.UK + l3) * SO;
     const int gs = tr.gs, ge = tr.ge;
     auto f = [&](int n) { return T.ws[n] * ((Fa[n] * Fb[n]) * Uu[n]); };
@@ -1128,7 +1142,10 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     const double dAC = dist(rc, J.atoms[Sa.atom]), dBC = dist(rc, J.atoms[Sb.atom]);
     const double za = J.pa[Sa.prim + pa], zb = J.pa[Sb.prim + pb];
     const double s1 = 2.0 * za * dAC, s2 = 2.0 * zb * dBC;
-    const double Cc = J.pd[Sa.prim + pa] * J.pd[Sb.prim + pb];
+    double Cc = J.pd[Sa.prim + pa] * J.pd[Sb.prim + pb];
+    // derivatives: times zeta^shift (type2.c:457-462)
+    for (int k = 0; k < tr.powa; k++) Cc *= za;
+    for (int k = 0; k < tr.powb; k++) Cc *= zb;
     const double zeta_p = za + zb;
     const Fm06 M = fm06(zeta_p, (za * dAC + zb * dBC) / zeta_p);
     const double minExp = T.lnAcc2;
@@ -1138,7 +1155,7 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     };
     const int brk = fm06_break(T, M, dAC, dBC, minExp, expo_of);
     const int end = brk - 1;
-    const int laC = Sa.l + G.l, lbC = Sb.l + G.l;
+    const int laC = tr.la + G.l, lbC = tr.lb + G.l;
     const ecp_gauss *gg = J.gauss + Cn.g0;
     auto eval = [&](int slot, int n) {
       double r = 0.0, w = 0.0, cu = 0.0, ex = 0.0;
@@ -1532,7 +1549,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
   const ecp_triplet tr = J.trip[t];
   const ecp_centre Cn = J.cent[tr.centre];
   const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
-  const int la = Sa.l, lb = Sb.l, lab = la + lb;
+  const int la = tr.la, lb = tr.lb, lab = la + lb;  // shifted for derivatives
   const int nca = ECP_DIM_C(la), ncb = ECP_DIM_C(lb);
   const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
   const double rAC[3] = {ra.x - rc.x, ra.y - rc.y, ra.z - rc.z};
@@ -1585,7 +1602,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
     const int *ea = T.ijk + 3 * r, *eb = T.ijk + 3 * q;
     return H[T.ijkIndex[((ea[0] + eb[0]) * D + ea[1] + eb[1]) * D + ea[2] + eb[2]]];
   };
-  shift_block(T, get1, la, lb, vA, vB, norm1, Jbuf, A.blocks + tr.blk1);
+  shift_block(T, get1, tr.oa, tr.ob, vA, vB, norm1, Jbuf, A.blocks + tr.blk1);
 }
 
 // Type-2 block of one triplet per wave: for each projector l, the T_l
@@ -1609,7 +1626,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm2(DevTables T, DevJob J, Asm
   const ecp_triplet tr = J.trip[t];
   const ecp_centre Cn = J.cent[tr.centre];
   const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
-  const int la = Sa.l, lb = Sb.l, lab = la + lb;
+  const int la = tr.la, lb = tr.lb, lab = la + lb;  // shifted for derivatives
   const int nca = ECP_DIM_C(la), ncb = ECP_DIM_C(lb);
   const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
   const double rAC[3] = {ra.x - rc.x, ra.y - rc.y, ra.z - rc.z};
@@ -1694,7 +1711,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm2(DevTables T, DevJob J, Asm
   }
   const double norm2 = (4.0 * M_PI) * (4.0 * M_PI);
   auto get2 = [&](int r, int q) { return blk[r * ncb + q]; };
-  shift_block(T, get2, la, lb, vA, vB, norm2, Jbuf, A.blocks + tr.blk2);
+  shift_block(T, get2, tr.oa, tr.ob, vA, vB, norm2, Jbuf, A.blocks + tr.blk2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1722,7 +1739,7 @@ struct JobLayout {
          i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, in_bytes = 0;
   size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
          d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
-  int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0;
+  int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0, deriv = 0;
 };
 
 struct Prepared {
@@ -2350,7 +2367,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   // type-1 units grouped by class lab, inside a class in triplet order
   for (int t = 0; t < nt; t++) {
     const ecp_triplet &tr = job->trip[t];
-    P.cls_count[job->shells[tr.sa].l + job->shells[tr.sb].l] += job->shells[tr.sa].K * job->shells[tr.sb].K;
+    P.cls_count[tr.la + tr.lb] += job->shells[tr.sa].K * job->shells[tr.sb].K;
   }
   for (int c = 0; c <= ECP_MAX_LAB; c++) P.cls_begin[c + 1] = P.cls_begin[c] + P.cls_count[c];
   std::vector<T1Unit> units1(P.cls_begin[ECP_MAX_LAB + 1]);
@@ -2360,7 +2377,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   for (int t = 0; t < nt; t++) {
     const ecp_triplet &tr = job->trip[t];
     const ecp_shell &Sa = job->shells[tr.sa], &Sb = job->shells[tr.sb];
-    const int c = Sa.l + Sb.l;
+    const int c = tr.la + tr.lb;
     t1first[t] = cls_fill[c];
     for (int pa = 0; pa < Sa.K; pa++)
       for (int pb = 0; pb < Sb.K; pb++) {
@@ -2374,17 +2391,17 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   const size_t nq = cls_qbase[ECP_MAX_LAB + 1];
   std::vector<int> tcls(nt);
   {
-    for (int t = 0; t < nt; t++) P.tcls_begin[job->shells[job->trip[t].sa].l + job->shells[job->trip[t].sb].l + 1]++;
+    for (int t = 0; t < nt; t++) P.tcls_begin[job->trip[t].la + job->trip[t].lb + 1]++;
     for (int c = 0; c <= ECP_MAX_LAB; c++) P.tcls_begin[c + 1] += P.tcls_begin[c];
     std::vector<int> fill(P.tcls_begin, P.tcls_begin + ECP_MAX_LAB + 1);
-    for (int t = 0; t < nt; t++) tcls[fill[job->shells[job->trip[t].sa].l + job->shells[job->trip[t].sb].l]++] = t;
+    for (int t = 0; t < nt; t++) tcls[fill[job->trip[t].la + job->trip[t].lb]++] = t;
   }
   std::vector<T2Group> groups;
   std::vector<int> goff, tgfirst(nt);
   int ntot2 = 0;
   for (int t = 0; t < nt; t++) {
     const ecp_triplet &tr = job->trip[t];
-    const int la = job->shells[tr.sa].l, lb = job->shells[tr.sb].l;
+    const int la = tr.la, lb = tr.lb;
     tgfirst[t] = (int)groups.size();
     for (int l = 0; l < job->cent[tr.centre].L; l++) {
       const int key = tmpl_key(la, lb, l);
@@ -2407,8 +2424,9 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   std::vector<int2> fitems, aitems;
   for (int c = 0; c < job->ncent; c++) {
     const ecp_centre &C = job->cent[c];
-    for (int s = 0; s < C.nsh; s++)
-      if (!job->win[C.win_off + s].skip) fitems.push_back(make_int2(c, s));
+    for (int run = 0; run <= job->deriv; run++)
+      for (int s = 0; s < C.nsh; s++)
+        if (!job->win[C.win_off + s].skip) fitems.push_back(make_int2(c, s * 4 + run));
     for (int a = 0; a < C.nat; a++) aitems.push_back(make_int2(c, a));
   }
   P.nfit = (int)fitems.size();
@@ -2476,9 +2494,10 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.angL = job->angL;
   Ly.ang_lam = job->ang_lam;
   Ly.ang_c = job->ang_c;
+  Ly.deriv = job->deriv;
   {
     int LM = 0, LC = 1;
-    for (int i = 0; i < job->nsh; i++) LM = std::max(LM, job->shells[i].l);
+    for (int i = 0; i < job->nsh; i++) LM = std::max(LM, job->shells[i].l + job->deriv);
     for (int c = 0; c < job->ncent; c++) LC = std::max(LC, job->cent[c].L);
     P.max_lbs = LM;
     P.max_L = LC;
@@ -2596,6 +2615,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   J.angL = Ly.angL;
   J.ang_lam = Ly.ang_lam;
   J.ang_c = Ly.ang_c;
+  J.deriv = Ly.deriv;
   J.maxPairs = P.maxPairs;
   P.u1 = (const T1Unit *)(A + Ly.i_u1);
   P.q = (double *)(A + Ly.d_q);

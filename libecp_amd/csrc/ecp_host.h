@@ -45,6 +45,7 @@ typedef struct {
 /* ---- per-call job (flat arrays, host side) ---- */
 typedef struct {
   int nmol;
+  int deriv;                 /* derivative order n (0, 1, 2) */
   int nat, nsh, nprim, ncent, ngauss;
   ecp_xyz *atoms;            /* [nat] */
   int *atom_maxl;            /* [nat] max basis l on the atom */
@@ -61,7 +62,7 @@ typedef struct {
   size_t nblk;               /* doubles of output blocks */
   int maxK;                  /* largest contraction length */
   int maxL;                  /* largest ECP local label */
-  int FL;                    /* F rows per (centre, shell) slab */
+  int FL;                    /* F rows per (centre, shell, run) slab; runs 0..deriv */
   int UK;                    /* r-power rows of the Ut table */
   size_t nUL, nUt, nF, nang; /* table sizes (doubles) */
   int angL;                  /* l-dimension of the per-(centre, atom) angular table */
@@ -143,7 +144,7 @@ int ecp_mol_scan(const ecp_mol *m, ecp_mol_dims *d, char *err, size_t errlen);
 int ecp_static_init(ecp_static *st, int maxL, int maxAlpha, const int *ordering, int lmaxOrdering,
                     int largeGridOrder, double tolerance, double accuracy, char *err, size_t errlen);
 void ecp_static_free(ecp_static *st);
-int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *job, char *err,
+int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st, ecp_job *job, char *err,
                   size_t errlen);
 void ecp_job_free(ecp_job *job);
 int ecp_mol_copy(const ecp_mol *src, ecp_mol *dst, ecp_mol_dims *d, char *err, size_t errlen);

@@ -229,7 +229,14 @@ static int trip_push(trip_vec *tv, const ecp_triplet *t) {
   return 0;
 }
 
-int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *job, char *err,
+/* Derivative shifts of the basis angular momenta, in the reference's order
+   (libecp.c:243-250): first derivatives (+1,0) (-1,0) (0,+1) (0,-1); second
+   derivatives the ten pairs below. */
+static const int kShiftN[3] = {1, 4, 10};
+static const int kShiftA[3][10] = {{0}, {+1, -1, 0, 0}, {+2, -2, +1, +1, -1, -1, 0, 0, +1, 0}};
+static const int kShiftB[3][10] = {{0}, {0, 0, +1, -1}, {0, 0, +1, -1, +1, -1, +2, -2, 0, +1}};
+
+int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st, ecp_job *job, char *err,
                   size_t errlen) {
   memset(job, 0, sizeof(*job));
   ecp_mol_dims *dims = calloc((size_t)nmol, sizeof(ecp_mol_dims));
@@ -249,12 +256,23 @@ int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *
     if (dims[m].maxLBS > maxLBS) maxLBS = dims[m].maxLBS;
     if (dims[m].maxLECP > maxL) maxL = dims[m].maxLECP;
   }
-  if (maxL - 1 + maxLBS > st->omLambda || maxLBS > st->omAlpha || maxL > st->omL) {
+  if (deriv < 0 || deriv > 2) {
+    set_err(err, errlen, "derivative order must be 0, 1 or 2");
+    free(dims);
+    return -1;
+  }
+  if (maxLBS + deriv > ECP_MAX_LBS) {
+    set_err(err, errlen, "basis angular momentum plus derivative order exceeds this build (at most 5)");
+    free(dims);
+    return -1;
+  }
+  if (maxL - 1 + maxLBS + deriv > st->omLambda || maxLBS + deriv > st->omAlpha || maxL > st->omL) {
     set_err(err, errlen, "molecule exceeds the angular dimensions of the static tables");
     free(dims);
     return -1;
   }
   job->nmol = nmol;
+  job->deriv = deriv;
   job->nat = nat;
   job->nsh = nsh;
   job->nprim = nprim;
@@ -262,11 +280,11 @@ int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *
   job->ngauss = ngauss;
   job->nwin = nwin;
   job->maxL = maxL;
-  job->FL = maxL + maxLBS;          /* rows lambda = 0 .. L-1+l_s */
-  job->UK = 2 * maxLBS + 1;         /* r^k, k = 0 .. la+lb */
+  job->FL = maxL + maxLBS + deriv;          /* rows lambda = 0 .. L-1+l_s+run */
+  job->UK = 2 * (maxLBS + deriv) + 1;       /* r^k, k = 0 .. la+lb (shifted) */
   job->angL = maxL;
-  job->ang_lam = maxL + maxLBS;     /* lambda = 0 .. L-1+lmax_a */
-  job->ang_c = ECP_DIM_C(maxLBS);
+  job->ang_lam = maxL + maxLBS + deriv;     /* lambda = 0 .. L-1+lmax_a+n */
+  job->ang_c = ECP_DIM_C(maxLBS + deriv);
   job->atoms = calloc((size_t)nat + 1, sizeof(ecp_xyz));
   job->atom_maxl = calloc((size_t)nat + 1, sizeof(int));
   job->atom_shell0 = calloc((size_t)nat + 1, sizeof(int));
@@ -359,7 +377,7 @@ int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *
       Cn->Ut_off = (int)ut;
       ut += (size_t)maxL * job->UK * ECP_SMALL_ORDER;
       Cn->F_off = (int)fo;
-      fo += (size_t)dims[m].nsh * job->FL * ECP_SMALL_ORDER;
+      fo += (size_t)(deriv + 1) * dims[m].nsh * job->FL * ECP_SMALL_ORDER;
       Cn->win_off = w0 + (c - c0) * dims[m].nsh;
       Cn->atom0 = a0;
       Cn->nat = M->nat;
@@ -398,20 +416,40 @@ int ecp_job_build(const ecp_mol *mols, int nmol, const ecp_static *st, ecp_job *
               const ecp_window *Wb = &job->win[Cn->win_off + (sb - s0)];
               if (Wb->skip) continue;
               ecp_triplet t;
+              memset(&t, 0, sizeof(t));
               t.centre = c;
               t.sa = sa;
               t.sb = sb;
               t.gs = Wa->start > Wb->start ? Wa->start : Wb->start;
               t.ge = Wa->end > Wb->end ? Wa->end : Wb->end;
               if (!(t.gs < t.ge)) continue;
-              const int nb = ECP_DIM_IJK(job->shells[sa].l) * ECP_DIM_IJK(job->shells[sb].l);
-              t.blk1 = (int)blk;
-              t.blk2 = (int)(blk + nb);
-              blk += 2 * (size_t)nb;
-              if (trip_push(&tv, &t)) {
-                set_err(err, errlen, "out of host memory");
-                rc = -1;
-                break;
+              /* derivatives: A = B = C contributes nothing (libecp.c:305-306) */
+              if (deriv > 0 && gA == Cn->atom && gB == Cn->atom) continue;
+              const int la0 = job->shells[sa].l, lb0 = job->shells[sb].l;
+              /* one block pair per derivative shift (libecp.c:322-330) */
+              for (int k = 0; k < kShiftN[deriv] && !rc; k++) {
+                const int sha = kShiftA[deriv][k], shb = kShiftB[deriv][k];
+                if (la0 < -sha || lb0 < -shb || (deriv == 1 && Cn->atom == gA && sha != 0) ||
+                    (deriv == 1 && Cn->atom == gB && shb != 0))
+                  continue;
+                t.shifta = sha;
+                t.shiftb = shb;
+                t.la = la0 + sha;
+                t.lb = lb0 + shb;
+                t.powa = sha > 0 ? sha : 0;
+                t.powb = shb > 0 ? shb : 0;
+                /* second derivatives: the (+1, 0) and (0, +1) blocks are
+                   shifted back on the unshifted side (libecp.c:364-369) */
+                t.oa = (deriv == 2 && sha == 1 && shb == 0) ? la0 : t.la;
+                t.ob = (deriv == 2 && sha == 0 && shb == 1) ? lb0 : t.lb;
+                const int nb = ECP_DIM_IJK(t.oa) * ECP_DIM_IJK(t.ob);
+                t.blk1 = (int)blk;
+                t.blk2 = (int)(blk + nb);
+                blk += 2 * (size_t)nb;
+                if (trip_push(&tv, &t)) {
+                  set_err(err, errlen, "out of host memory");
+                  rc = -1;
+                }
               }
             }
           }

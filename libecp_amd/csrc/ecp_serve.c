@@ -207,7 +207,7 @@ static void *worker_main(void *arg) {
     while (!ok) {
       for (int i = 0; i < n; i++) mols[i] = rs[i]->mol;
       err[0] = 0;
-      if (ecp_job_build(mols, n, &s->st, &job, err, sizeof(err))) {
+      if (ecp_job_build(mols, n, 0, &s->st, &job, err, sizeof(err))) {
         bytes = (size_t)-1;
       } else if (ecp_dev_plan(w->dev, &job, &bytes, err, sizeof(err))) {
         ecp_job_free(&job);

@@ -75,13 +75,19 @@ class Handle:
                                C.c_int(lmax), op, C.c_int(grid), C.c_double(tol), C.c_double(acc))
         if not self.h:
             raise RuntimeError("libECP_init failed: " + last_error())
+        self.n = n
 
     def blocks(self):
         """Runs calculateECPIntegrals and returns (rc, keys, blocks)."""
         keys, blocks = [], []
+        order = self.n
 
         def cb(A, sa, la, shifta, B, sb, lb, shiftb, Cc, I, args):
-            n = _ijk(la + shifta) * _ijk(lb + shiftb)
+            # IJK(la+shifta) x IJK(lb+shiftb); for second derivatives the
+            # (1, 0) and (0, 1) blocks keep the unshifted l (libecp.c:364-369)
+            las = la if (order == 2 and (shifta, shiftb) == (1, 0)) else la + shifta
+            lbs = lb if (order == 2 and (shifta, shiftb) == (0, 1)) else lb + shiftb
+            n = _ijk(las) * _ijk(lbs)
             keys.append((A, sa, la, shifta, B, sb, lb, shiftb, Cc))
             blocks.append(np.ctypeslib.as_array(I, shape=(n,)).copy())
         f = CB(cb)

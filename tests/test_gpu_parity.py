@@ -67,6 +67,39 @@ def test_against_reference_fixtures(path):
     assert np.isfinite(up).all()
 
 
+@pytest.mark.parametrize("path", [p for p in GOLDEN if "_n1" in p or "_n2" in p],
+                         ids=[i for i in golden_ids() if i.endswith(("_n1", "_n2"))])
+def test_derivative_fixtures(path):
+    """Derivative integrals (n = 1, 2): every shifted block of the reference's
+    callback stream."""
+    g = load_golden(path)
+    rc, keys, blocks = gpu_blocks(g["mol"], n=g["n"])
+    assert rc == g["rc"]
+    assert keys == g["keys"]
+    for i, (k, a, b) in enumerate(zip(keys, blocks, g["blocks"])):
+        if np.isfinite(b).all():
+            ok, excess = within(a, b, ATOL, RTOL)
+            assert ok, (k, 1 + i % 2, excess)
+
+
+@pytest.mark.parametrize("n", [1, 2])
+@pytest.mark.parametrize("seed", [0, 3])
+def test_derivatives_against_oracle(n, seed):
+    m = plain_small(seed)
+    rc, keys, blocks = gpu_blocks(m, n=n)
+    rc2, col = E.oracle_blocks(m, n=n, options=1)
+    assert rc == rc2 == 0
+    check_stream(keys, blocks, col.keys, col.blocks)
+
+
+def test_derivatives_medium_cluster_against_oracle():
+    m = medium_cluster(0, nat=16, n_ecp=4, box=14.0)
+    rc, keys, blocks = gpu_blocks(m, n=1)
+    rc2, col = E.oracle_blocks(m, n=1, options=1)
+    assert rc == rc2 == 0
+    check_stream(keys, blocks, col.keys, col.blocks)
+
+
 def test_minimal_ecp_type1_exact():
     """L = 1 with an s basis: the reference reads past its factorial table
     (NaN); the device result is checked against the exact three-centre

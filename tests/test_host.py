@@ -48,9 +48,25 @@ def handle(m, **kw):
     return h
 
 
-def test_init_rejects_derivatives_with_message():
-    assert not handle(plain_small(0), n=1)
+def test_init_rejects_derivative_order_above_two():
+    assert not handle(plain_small(0), n=3)
     assert "derivative" in native.last_error()
+    assert not handle(plain_small(0), n=-1)
+
+
+def test_derivative_order_limited_by_basis_l():
+    # basis up to d (l = 2): n = 2 reaches l = 4 and is accepted; an h shell
+    # (l = 5) leaves no room for a shift
+    m = plain_small(0)
+    keys, _ = screened(m, n=2)
+    assert len(keys) > 0
+    m.lBS[-1] = 5
+    lib = native.lib()
+    h = handle(m, n=1)
+    assert h
+    assert lib.libecp_screened_triplets(C.c_void_p(h), None, 0) == -1
+    assert "derivative" in native.last_error()
+    lib.libECP_free(C.c_void_p(h))
 
 
 def test_init_rejects_local_only_ecp():
@@ -76,9 +92,9 @@ def test_init_rejects_short_custom_ordering():
     assert not handle(plain_small(0), lmax=2, ordering=ordering.ctypes.data_as(C.POINTER(C.c_int)))
 
 
-def screened(m):
+def screened(m, n=0):
     lib = native.lib()
-    h = handle(m)
+    h = handle(m, n=n)
     assert h
     n = lib.libecp_screened_triplets(C.c_void_p(h), None, 0)
     keys = np.zeros(5 * max(n, 1), dtype=np.int32)
@@ -105,6 +121,26 @@ def test_screening_matches_oracle_seeds(seed):
     rc, col = E.oracle_blocks(m, options=1)
     assert keys == [(k[8], k[0], k[1], k[4], k[5]) for k in col.keys[::2]]
     assert nominal == m.nominal_triplets() == 78
+
+
+@pytest.mark.parametrize("n", [1, 2])
+@pytest.mark.parametrize("seed", range(3))
+def test_derivative_screening_matches_oracle(n, seed):
+    # one record per (triplet, shift); A = B = C triplets and the shifts the
+    # reference skips (libecp.c:304-330) are absent
+    m = plain_small(seed)
+    keys, _ = screened(m, n=n)
+    rc, col = E.oracle_blocks(m, n=n, options=1)
+    assert keys == [(k[8], k[0], k[1], k[4], k[5]) for k in col.keys[::2]]
+
+
+@pytest.mark.parametrize("name", ["collinear", "one_centre", "two_centres"])
+def test_derivative_screening_matches_oracle_edge_cases(name):
+    m = CASES[name]()
+    for n in (1, 2):
+        keys, _ = screened(m, n=n)
+        rc, col = E.oracle_blocks(m, n=n, options=1)
+        assert keys == [(k[8], k[0], k[1], k[4], k[5]) for k in col.keys[::2]]
 
 
 def test_screening_matches_oracle_medium_cluster():
