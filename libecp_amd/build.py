@@ -57,6 +57,30 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+ARCHIVE = os.path.join(PKG, "libecp_amd.a")
+EXAMPLE_SRC = os.path.join(ROOT, "tests", "c", "example_getintegrals.c")
+EXAMPLES = {"shared": os.path.join(ROOT, "tests", "c", "example_getintegrals"),
+            "static": os.path.join(ROOT, "tests", "c", "example_getintegrals_static")}
+
+
+def build_examples() -> None:
+    """The static archive libecp_amd.a and a C caller of getIntegrals() linked
+    both ways, as a program that used the reference's libecp.a would link:
+    against the shared library (rpath to it) and against the archive."""
+    objs = [os.path.join(BUILD, s.replace(".c", ".o")) for s in C_SOURCES] + \
+           [os.path.join(BUILD, s.replace(".hip", ".o")) for s in HIP_SOURCES]
+    if os.path.exists(ARCHIVE):
+        os.remove(ARCHIVE)
+    _run(["ar", "rcs", ARCHIVE, *objs])
+    inc = os.path.join(ROOT, "include")
+    eo = os.path.join(BUILD, "example_getintegrals.o")
+    _run(["gcc", "-O2", "-std=gnu11", "-I" + inc, "-c", EXAMPLE_SRC, "-o", eo])
+    _run(["gcc", eo, "-o", EXAMPLES["shared"], "-L" + PKG, "-lecp_amd",
+          "-Wl,-rpath,$ORIGIN/../../libecp_amd", "-lm"])
+    _run([HIPCC, "--hip-link", "--offload-arch=" + ARCH, eo, ARCHIVE, "-o", EXAMPLES["static"], "-lm",
+          "-lpthread"])
+
+
 def build_oracle(with_reference: bool = True) -> None:
     """Compiles oracle/liboracle.so and, where the reference's sources are
     present, oracle/_ref/ (checkers for tests and the CPU baseline)."""
@@ -68,4 +92,5 @@ def build_oracle(with_reference: bool = True) -> None:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    build_examples()
     build_oracle()
