@@ -230,6 +230,72 @@ __device__ void real_sph_inplace(int lmax, double theta, double phi, const doubl
 #undef RSH
 }
 
+// Column m >= 0 of real_sph_inplace (entries (l, +-m), l = m..lmax) with
+// the same arithmetic, so that the columns of one direction can be formed by
+// different lanes: s1^m by repeated multiplication, the diagonal, the first
+// subdiagonal and the three-term recurrence in l, the normalisation per
+// (l, m), and sin/cos(m phi) by the angle-addition recurrence from m = 1.
+__device__ void real_sph_column(int lmax, int m, double theta, double phi, const double *__restrict__ fac,
+                                const double *__restrict__ dfac, double *out) {
+#define RSH(l, mm) ((l) * (l) + (l) + (mm))
+  const double x = cos(theta);
+  if (1.0 == x || -1.0 == x) {
+    for (int l = m; l <= lmax; l++) {
+      out[RSH(l, m)] = 0.0;
+      out[RSH(l, -m)] = 0.0;
+    }
+    if (m == 0) {
+      double v = 1.0;
+      for (int l = 0; l <= lmax; l++) {
+        if (l > 0 && -1.0 == x) v = -v;
+        out[RSH(l, 0)] = v;
+      }
+    }
+  } else {
+    const double s1 = sqrt(1.0 - x * x);
+    // P_mm, P_(m+1)m, then the recurrence
+    double pmm;
+    if (m == 0) {
+      pmm = 1.0;
+    } else {
+      double sl = s1;
+      for (int l = 2; l <= m; l++) sl = sl * s1;
+      pmm = sl * dfac[2 * m - 1];
+    }
+    out[RSH(m, m)] = pmm;
+    if (m + 1 <= lmax) out[RSH(m + 1, m)] = x * (2 * m + 1) * out[RSH(m, m)];
+    for (int l = m + 2; l <= lmax; l++)
+      out[RSH(l, m)] = (x * (2 * l - 1) * out[RSH(l - 1, m)] - (l + m - 1) * out[RSH(l - 2, m)]) / (l - m);
+  }
+  const double sp = sin(phi), cp = cos(phi);
+  double sm = 0.0, cm = 1.0;
+  for (int k = 1; k <= m; k++) {
+    if (0.0 == phi) {
+      sm = 0.0;
+      cm = 1.0;
+    } else if (k == 1) {
+      sm = sp;
+      cm = cp;
+    } else {
+      const double ns = sp * cm + cp * sm;
+      cm = cp * cm - sp * sm;
+      sm = ns;
+    }
+  }
+  for (int l = m; l <= lmax; l++) {
+    const double norm0 = sqrt((2.0 * l + 1.0) / (2.0 * M_PI));
+    if (m == 0) {
+      out[RSH(l, 0)] = norm0 * out[RSH(l, 0)];
+      out[RSH(l, 0)] = out[RSH(l, 0)] / sqrt(2.0);
+    } else {
+      const double P = (sqrt(fac[l - m] / fac[l + m]) * norm0) * out[RSH(l, m)];
+      out[RSH(l, -m)] = P * sm;
+      out[RSH(l, m)] = P * cm;
+    }
+  }
+#undef RSH
+}
+
 // (r, theta, phi) of util.c:74-106
 __device__ void sph_coords(double x, double y, double z, double &r, double &theta, double &phi) {
   const double acc = 1.0E-14;
@@ -1455,7 +1521,7 @@ struct AsmArgs {
   double *blocks;
   // dynamic LDS layout (doubles): J is shared; the type-1 region (rsph, H)
   // and the type-2 region (blk, T) overlap
-  int lds_blk, lds_H, lds_J, lds_rsph, lds_T, lds_rs_stride, pair_chunk;
+  int lds_blk, lds_H, lds_J, lds_rsph, lds_T, lds_rs_stride, pair_chunk, lds_Q;
 };
 
 // binomial shift of a block over C-centred monomials to the IJK(la) x IJK(lb)
@@ -1542,6 +1608,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
   double *H = lds + A.lds_H;
   double *Jbuf = lds + A.lds_J;
   double *rsph = lds + A.lds_rsph;
+  double *Qs = lds + A.lds_Q;
   const int t = tlist[blockIdx.x];
   // filter 1: only triplets without large-grid work, 2: only those with it
   if ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t])) return;
@@ -1561,20 +1628,28 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
   (void)ncb;
 
   const int np = ECP_DIM_C(lab);
+  const int nql = nq_of(lab);
   const int npairs = Sa.K * Sb.K;
   const int u0 = A.t1first[t];
   int qstart = 0;
   for (int idx = threadIdx.x; idx < np; idx += blockDim.x) H[idx] = 0.0;
   for (int c0 = 0; c0 < npairs; c0 += A.pair_chunk) {
     const int nc = min(A.pair_chunk, npairs - c0);
-    if ((int)threadIdx.x < nc) {
-      const T1Unit U = A.units1[u0 + c0 + threadIdx.x];
+    // harmonics of the chunk's pair directions, one (pair, m) column per lane;
+    // the pairs' Q values staged alongside
+    for (int it = threadIdx.x; it < nc * (lab + 1); it += blockDim.x) {
+      const int k = it / (lab + 1), m = it - k * (lab + 1);
+      const T1Unit U = A.units1[u0 + c0 + k];
       const double za = J.pa[Sa.prim + U.pa], zb = J.pa[Sb.prim + U.pb];
       double P[3];
       for (int p = 0; p < 3; p++) P[p] = 2.0 * (za * rAC[p] + zb * rBC[p]);
       double r, th, ph;
       sph_coords(P[0], P[1], P[2], r, th, ph);
-      real_sph_inplace(lab, th, ph, A.fac, A.dfac, rsph + threadIdx.x * A.lds_rs_stride);
+      real_sph_column(lab, m, th, ph, A.fac, A.dfac, rsph + k * A.lds_rs_stride);
+    }
+    for (int it = threadIdx.x; it < nc * nql; it += blockDim.x) {
+      const int k = it / nql, e = it - k * nql;
+      Qs[k * nql + e] = A.qout[A.units1[u0 + c0 + k].qoff + e];
     }
     __syncthreads();
     for (int p = threadIdx.x; p < np; p += blockDim.x) {
@@ -1586,7 +1661,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
       double acc = H[p];
       for (int k = 0; k < nc; k++) {
         const double *rs = rsph + k * A.lds_rs_stride;
-        const double *Q = A.qout + A.units1[u0 + c0 + k].qoff + qstart;
+        const double *Q = Qs + k * nql + qstart;
         for (int l = deg; l >= 0; l -= 2) {
           double f = 0.0;
           for (int m = 0; m < 2 * l + 1; m++) f += rs[ECP_IDX_LM(l, m)] * pm[ECP_IDX_LM(l, m)];
@@ -2509,6 +2584,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     const int base = o;
     aa.lds_rsph = o; o += aa.pair_chunk * aa.lds_rs_stride;
     aa.lds_H = o; o += ECP_DIM_C(2 * LM);
+    aa.lds_Q = o; o += aa.pair_chunk * nq_of(2 * LM);
     const int end1 = o;
     o = base;
     aa.lds_blk = o; o += ECP_DIM_C(LM) * ECP_DIM_C(LM);
