@@ -626,9 +626,10 @@ __global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Uni
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
 // Tail: one wave per primitive pair left open by the head, levels [head,
-// nlevels).  Lane k evaluates the k-th point pair of the level (at most 64
-// pairs per level of the 383-point grid) for every Q entry and stores the
-// pair sum T_k = (0 + f(left)) + f(right) in LDS; lane e then adds the T_k
+// nlevels).  Lane k evaluates the k-th point pair of a round of levels (at
+// most 64 pairs per round; a level of the 383-point grid has at most 64) for
+// every Q entry and stores the pair sum T_k = (0 + f(left)) + f(right) in
+// LDS; lane e then adds each level's T_k
 // to its entry's I in pair order and applies the PS93 test, so the
 // arithmetic is that of the lane-serial integrator.
 constexpr int kTS = 65;  // LDS stride of the per-entry pair sums (bank spread)
@@ -683,10 +684,17 @@ __device__ __forceinline__ void t1tail_body(int bid, int nb, const DevTables &T,
       I = state[2 * plane + U.qoff + lane];
       mydone = (done0 >> lane) & 1ull;
     }
-    for (int lv = head; lv < T.s93.nlevels; lv++) {
-      const int b0 = T.s93.lvl_begin[lv], np = T.s93.lvl_begin[lv + 1] - b0;
+    // rounds of consecutive levels whose point pairs fit the wave together
+    // (levels 8-11: 48 pairs, 12-13: 64, 14: 64); the sums and tests then
+    // follow level by level, so only the evaluation is speculative
+    int lv = head;
+    while (lv < T.s93.nlevels) {
+      const int b0 = T.s93.lvl_begin[lv];
+      int l1 = lv + 1;
+      while (l1 < T.s93.nlevels && T.s93.lvl_begin[l1 + 1] - b0 <= 64) l1++;
+      const int tot = T.s93.lvl_begin[l1] - b0;
       int cnt = 0;
-      if (lane < np) {
+      if (lane < tot) {
         const int li = T.p93l[b0 + lane], ri = T.p93r[b0 + lane];
         double t[NQ];
 #pragma unroll
@@ -702,27 +710,38 @@ __device__ __forceinline__ void t1tail_body(int bid, int nb, const DevTables &T,
 #pragma unroll
         for (int e = 0; e < NQ; e++) s_T[e * kTS + lane] = t[e];
       }
-      const bool any = wave_any(cnt > 0);
+      const unsigned long long cmask = __ballot(cnt > 0);
       __syncthreads();
-      if (lane < NQ)
-        for (int k = 0; k < np; k++) I += s_T[lane * kTS + k];
-      __syncthreads();
-      if (!mydone) {
-        const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
-        double err;
-        if (j == 0) {
-          p += (I - q);
-          err = 16 * fabs(q - 3 * p / 2) / (3 * nn);
-        } else {
-          err = 16 * fabs(I - 2 * q) / (3 * nn);
-          q = I;
+      bool stop = false;
+      for (int l = lv; l < l1; l++) {
+        const int o = T.s93.lvl_begin[l] - b0, np = T.s93.lvl_begin[l + 1] - T.s93.lvl_begin[l];
+        const unsigned long long lm = (np >= 64) ? ~0ull : (((1ull << np) - 1ull) << o);
+        const bool any = (cmask & lm) != 0ull;
+        if (lane < NQ)
+          for (int k = 0; k < np; k++) I += s_T[lane * kTS + o + k];
+        if (!mydone) {
+          const int j = T.s93.lvl_j[l], nn = T.s93.lvl_n[l];
+          double err;
+          if (j == 0) {
+            p += (I - q);
+            err = 16 * fabs(q - 3 * p / 2) / (3 * nn);
+          } else {
+            err = 16 * fabs(I - 2 * q) / (3 * nn);
+            q = I;
+          }
+          if (any && err < T.tol) {
+            mydone = true;
+            p = 16 * q / (3 * nn);
+          }
         }
-        if (any && err < T.tol) {
-          mydone = true;
-          p = 16 * q / (3 * nn);
+        if (!wave_any(!mydone)) {
+          stop = true;
+          break;
         }
       }
-      if (!wave_any(!mydone)) break;
+      __syncthreads();
+      if (stop) break;
+      lv = l1;
     }
     if (lane < NQ) qout[U.qoff + lane] = mydone ? p : 0.0;
     const unsigned long long failed = __ballot(lane < NQ && !mydone);
