@@ -24,6 +24,19 @@ int libecp_screened_triplets(libECPHandle *h, int *keys, int cap);
    screening (libecp.c:256-312), the unit of the throughput metric. */
 long long libecp_nominal_triplets(libECPHandle *h);
 
+/* ---- density contraction on the device ----------------------------------
+   The consumer sketched in getIntegrals.c:15-18 (a density matrix and a
+   gradient in place of the AO matrix), evaluated where the blocks are: no
+   block leaves the device and no callback runs.  P is the full symmetric
+   density (nao x nao, row stride rowdim) in getIntegrals' AO order (atoms,
+   their shells, the shells' Cartesian components).  A handle of order
+   n = 0 gives out[0] = E = sum_{mu nu} P_{mu nu} V_{mu nu}; n = 1 gives
+   out[3 A + d] = dE/dR_{A,d} for every atom A (basis functions and ECP
+   centre move with their atom).  Returns 0, 1 / 2 when a type-1 / type-2
+   integration failed (out is then not meaningful), or -1
+   (libecp_last_error(); also for n = 2). */
+int libecp_contract(libECPHandle *h, int rowdim, const double *P, double *out);
+
 /* ---- batched evaluation ------------------------------------------------
    Many molecules in one resident job: one upload, one kernel sequence over
    the triplets of every molecule (replayed from a captured HIP graph), one
@@ -44,6 +57,12 @@ typedef struct libecp_batch libecp_batch;
    (libecp_last_error()). */
 libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int largeGridOrder, double tolerance,
                                   double accuracy);
+/* The same for derivative order n (0, 1 or 2; libecp.c:246-250): the
+   blocks of the shifted shells, as a handle of order n delivers them.  A
+   batch of order n > 0 has no ECP matrix (libecp_batch_matrix fails); its
+   use is libecp_batch_contract (n = 1). */
+libecp_batch *libecp_batch_create_n(const libecp_molecule *mols, int nmol, int n, int largeGridOrder,
+                                    double tolerance, double accuracy);
 /* Evaluates the whole batch `repeats` times on the device and waits;
    *ms = device time of one evaluation.  0 on success, -1 on a device error. */
 int libecp_batch_execute(libecp_batch *b, int repeats, double *ms);
@@ -53,6 +72,16 @@ int libecp_batch_fetch(libecp_batch *b);
    rowdim), as getIntegrals does; returns 0, or 1 / 2 when a type-1 / type-2
    integration of that molecule failed (blocks up to the failure added). */
 int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I);
+/* Contracts the last evaluation with one density per molecule on the
+   device (see libecp_contract): molecule m's full symmetric nao_m x nao_m
+   density, row-major, at P + sum_{i<m} nao_i^2.  out: n = 0: one energy per
+   molecule; n = 1: 3 nrAtoms_m gradient components per molecule, molecules
+   in order.  codes[m] (codes may be NULL): 0, or 1 / 2 when a type-1 /
+   type-2 integration of molecule m failed (its out entries are then not
+   meaningful).  device != 0: P and out are device pointers on the batch's
+   device (no host copies).  *ms (ms may be NULL): device time of the
+   contraction kernels.  0, or -1. */
+int libecp_batch_contract(libecp_batch *b, const double *P, double *out, int *codes, int device, double *ms);
 /* Nominal (pre-screening) and evaluated triplet counts of the batch. */
 long long libecp_batch_triplets(libecp_batch *b, long long *evaluated);
 /* Stage timings (ms) of the first evaluation: tables, quadratures (small

@@ -219,6 +219,53 @@ int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *
 }
 
 /* ---------------------------------------------------------------------- */
+/* Density contraction on the device (include/libecp_ext.h).              */
+
+int libecp_contract(libECPHandle *h, int rowdim, const double *P, double *out) {
+  char err[512] = "";
+  if (!h || !P || !out) {
+    set_last_error("NULL argument");
+    return -1;
+  }
+  if (h->n > 1) {
+    set_last_error("density contraction supports derivative orders 0 and 1");
+    return -1;
+  }
+  if (h->job_ready) {
+    ecp_job_free(&h->job);
+    h->job_ready = 0;
+  }
+  if (ecp_job_build(&h->mol, 1, h->n, &h->st, &h->job, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  h->job_ready = 1;
+  int nao = 0;
+  for (int s = 0; s < h->job.nsh; s++) nao += ECP_DIM_IJK(h->job.shells[s].l);
+  if (rowdim < nao) {
+    set_last_error("rowdim is smaller than the number of basis functions");
+    return -1;
+  }
+  const int nout = h->n == 0 ? 1 : 3 * h->nat;
+  if (h->job.ntrip == 0) {
+    for (int i = 0; i < nout; i++) out[i] = 0.0;
+    return 0;
+  }
+  if (ensure_device(h)) return -1;
+  const long long poff = 0;
+  const int prow = rowdim;
+  const size_t nP = nao ? (size_t)rowdim * (nao - 1) + nao : 0;
+  int code = 0;
+  if (ecp_dev_prepare(h->dev, &h->job, err, sizeof(err)) ||
+      ecp_dev_execute(h->dev, 1, NULL, err, sizeof(err)) ||
+      ecp_dev_contract(h->dev, &h->job, P, nP, 0, &poff, &prow, out, 0, &code, NULL, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  return code;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Additive queries (include/libecp_ext.h).                               */
 
 int libecp_screened_triplets(libECPHandle *h, int *keys, int cap) {
@@ -260,7 +307,8 @@ long long libecp_nominal_triplets(libECPHandle *h) {
 /* Batched evaluation (include/libecp_ext.h).                              */
 
 struct libecp_batch {
-  int nmol;
+  int nmol, n;
+  int *nao, *nat;  /* per molecule */
   ecp_mol *mols; /* private copies */
   ecp_static st;
   ecp_dev *dev;
@@ -279,6 +327,8 @@ void libecp_batch_free(libecp_batch *b) {
   for (int i = 0; i < b->nmol; i++) ecp_mol_release(&b->mols[i]);
   free(b->mols);
   free(b->aoff);
+  free(b->nao);
+  free(b->nat);
   free(b->blocks);
   free(b->status);
   free(b);
@@ -286,14 +336,26 @@ void libecp_batch_free(libecp_batch *b) {
 
 libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int largeGridOrder, double tolerance,
                                   double accuracy) {
+  return libecp_batch_create_n(mols, nmol, 0, largeGridOrder, tolerance, accuracy);
+}
+
+libecp_batch *libecp_batch_create_n(const libecp_molecule *mols, int nmol, int n, int largeGridOrder,
+                                    double tolerance, double accuracy) {
   char err[512] = "";
   set_last_error("");
   if (nmol <= 0 || !mols) {
     set_last_error("empty batch");
     return NULL;
   }
+  if (n < 0 || n > 2) {
+    set_last_error("derivative order must be 0, 1 or 2");
+    return NULL;
+  }
   libecp_batch *b = calloc(1, sizeof(*b));
   b->nmol = nmol;
+  b->n = n;
+  b->nao = calloc((size_t)nmol, sizeof(int));
+  b->nat = calloc((size_t)nmol, sizeof(int));
   b->mols = calloc((size_t)nmol, sizeof(ecp_mol));
   int maxL = 1, maxLBS = 0;
   for (int i = 0; i < nmol; i++) {
@@ -309,11 +371,13 @@ libecp_batch *libecp_batch_create(const libecp_molecule *mols, int nmol, int lar
       return NULL;
     }
     b->nominal += (long long)d.necp * d.nsh * (d.nsh + 1) / 2;
+    b->nao[i] = d.nao;
+    b->nat[i] = M->nrAtoms;
     if (d.maxLECP > maxL) maxL = d.maxLECP;
     if (d.maxLBS > maxLBS) maxLBS = d.maxLBS;
   }
-  if (ecp_static_init(&b->st, maxL, maxLBS, NULL, -1, largeGridOrder, tolerance, accuracy, err, sizeof(err)) ||
-      ecp_job_build(b->mols, nmol, 0, &b->st, &b->job, err, sizeof(err))) {
+  if (ecp_static_init(&b->st, maxL, maxLBS + n, NULL, -1, largeGridOrder, tolerance, accuracy, err, sizeof(err)) ||
+      ecp_job_build(b->mols, nmol, n, &b->st, &b->job, err, sizeof(err))) {
     set_last_error(err);
     libecp_batch_free(b);
     return NULL;
@@ -353,7 +417,41 @@ int libecp_batch_fetch(libecp_batch *b) {
 
 int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I) {
   if (!b || mol < 0 || mol >= b->nmol) return -1;
+  if (b->n != 0) {
+    set_last_error("the ECP matrix of a derivative batch is not defined (use libecp_batch_contract)");
+    return -1;
+  }
   return ecp_job_scatter(&b->job, mol, b->aoff, b->blocks, b->status, rowdim, 0, I);
+}
+
+int libecp_batch_contract(libecp_batch *b, const double *P, double *out, int *codes, int device, double *ms) {
+  char err[512] = "";
+  if (!b || !P || !out) {
+    set_last_error("NULL argument");
+    return -1;
+  }
+  if (b->n > 1) {
+    set_last_error("density contraction supports derivative orders 0 and 1");
+    return -1;
+  }
+  long long *poff = malloc(sizeof(long long) * (size_t)b->nmol);
+  int *prow = malloc(sizeof(int) * (size_t)b->nmol);
+  int *cd = codes ? codes : malloc(sizeof(int) * (size_t)b->nmol);
+  size_t nP = 0;
+  for (int m = 0; m < b->nmol; m++) {
+    poff[m] = (long long)nP;
+    prow[m] = b->nao[m];
+    nP += (size_t)b->nao[m] * b->nao[m];
+  }
+  int rc = 0;
+  if (ecp_dev_contract(b->dev, &b->job, P, nP, device, poff, prow, out, device, cd, ms, err, sizeof(err))) {
+    set_last_error(err);
+    rc = -1;
+  }
+  free(poff);
+  free(prow);
+  if (!codes) free(cd);
+  return rc;
 }
 
 long long libecp_batch_triplets(libecp_batch *b, long long *evaluated) {

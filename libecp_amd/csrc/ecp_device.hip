@@ -1914,6 +1914,10 @@ struct ecp_dev {
   std::vector<hipEvent_t> pev;
   std::vector<int> pfam;
   int npev = 0;
+  // density contraction: index tables, densities, partial sums, results
+  char *con_mem = nullptr;
+  size_t con_cap = 0;
+  hipEvent_t con_ev[2] = {};
 };
 
 namespace {
@@ -2116,6 +2120,9 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   if (d->t2done) (void)hipEventDestroy(d->t2done);
   if (d->t1done) (void)hipEventDestroy(d->t1done);
   for (auto e : d->pev) (void)hipEventDestroy(e);
+  if (d->con_mem) (void)hipFree(d->con_mem);
+  for (int i = 0; i < 2; i++)
+    if (d->con_ev[i]) (void)hipEventDestroy(d->con_ev[i]);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -2858,6 +2865,258 @@ extern "C" int ecp_dev_run(ecp_dev *d, const ecp_job *job, double *blocks_out, u
   if (ecp_dev_prepare(d, job, err, errlen)) return -1;
   if (ecp_dev_execute(d, 1, nullptr, err, errlen)) return -1;
   return ecp_dev_fetch(d, blocks_out, status_out, err, errlen);
+}
+
+// ---------------------------------------------------------------------------
+// Density contraction of the evaluated blocks on the device, the consumer
+// that getIntegrals.c:15-18 anticipates (a density P and a gradient in
+// place of the AO matrix): no blocks leave the device.
+//   n = 0: E_m = sum_{mu nu} P_{mu nu} V_{mu nu}; a block of shells a != b
+//          stands for itself and its transpose (weight 2), a diagonal shell
+//          block (a = b) for itself (weight 1).
+//   n = 1: dE_m/dR_{X,d}.  A primitive's derivative is
+//          d/dA_d g(a) = 2 zeta g(a + 1_d) - a_d g(a - 1_d) (libecp.c:242-245),
+//          whose zeta factor the (+1) blocks already carry; the potential's
+//          own derivative follows from translational invariance,
+//          d/dC = -(d/dA + d/dB), so each block's contribution s goes to the
+//          atom of its shifted shell with +s and to the ECP centre with -s.
+//          Blocks the screening skips (A = B = C; the side on the centre,
+//          libecp.c:305-306,326-330) would cancel in that sum.
+// Partial sums per triplet are reduced by a fixed butterfly, and the
+// per-molecule / per-atom sums run over triplets in a fixed order, so
+// results do not depend on scheduling.
+
+namespace {
+struct ConArgs {
+  const ecp_triplet *trip;
+  const ecp_shell *shells;
+  const ecp_centre *cent;
+  const double *blocks;
+  const unsigned char *stat;
+  const int *aoff;        // AO offset of each global shell within its molecule
+  const double *P;        // densities, packed by molecule
+  const long long *poff;  // [nmol] offset of each molecule's density
+  const int *prow;        // [nmol] its row stride
+  const int *tmb;         // [nmol + 1] triplet range of each molecule
+  const int *umol;        // [nunits] molecule of each output unit
+  const int *uatom;       // [nunits] global atom of each output unit (n = 1)
+  const int *uout;        // [nunits] output offset of each unit
+  double *part;           // [ntrip * nd] weighted partial sums
+  double *out;
+  int *codes;             // [nmol] first failed integration: 0, 1 or 2
+  int ntrip, nd, nunits, nmol;
+};
+
+__device__ __forceinline__ double wave_sum_fixed(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(64) void k_con_part(DevTables T, ConArgs A) {
+  const int lane = threadIdx.x;
+  const int D = T.ijkDim;
+  for (int t = blockIdx.x; t < A.ntrip; t += gridDim.x) {
+    const ecp_triplet tr = A.trip[t];
+    const ecp_shell Sa = A.shells[tr.sa], Sb = A.shells[tr.sb];
+    const double *__restrict__ Pm = A.P + A.poff[Sa.mol];
+    const int R = A.prow[Sa.mol];
+    const int a0 = A.aoff[tr.sa], b0 = A.aoff[tr.sb];
+    const int la = Sa.l, lb = Sb.l, nb = ECP_DIM_IJK(lb), nab = ECP_DIM_IJK(la) * nb;
+    const int no = ECP_DIM_IJK(tr.ob);
+    const double *__restrict__ b1 = A.blocks + tr.blk1;
+    const double *__restrict__ b2 = A.blocks + tr.blk2;
+    const bool bra = tr.shifta != 0;
+    const int sh = bra ? tr.shifta : tr.shiftb, ls = bra ? la : lb;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int e = lane; e < nab; e += 64) {
+      const int i = e / nb, j = e - i * nb;
+      const double p = Pm[(size_t)(a0 + i) * R + b0 + j];
+      if (A.nd == 1) {
+        acc[0] += p * (b1[i * no + j] + b2[i * no + j]);
+        continue;
+      }
+      const int *ex = T.ijk + 3 * ECP_IDX_C(ls, bra ? i : j);
+#pragma unroll
+      for (int dd = 0; dd < 3; dd++) {
+        int x[3] = {ex[0], ex[1], ex[2]};
+        double coef;
+        if (sh > 0) {
+          coef = 2.0;
+          x[dd] += 1;
+        } else {
+          if (x[dd] == 0) continue;
+          coef = -(double)x[dd];
+          x[dd] -= 1;
+        }
+        const int c = T.ijkIndex[(x[0] * D + x[1]) * D + x[2]] - ECP_IDX_C(ls + sh, 0);
+        const int o = bra ? c * no + j : i * no + c;
+        acc[dd] += coef * (p * (b1[o] + b2[o]));
+      }
+    }
+    const double w = (tr.sa == tr.sb) ? 1.0 : 2.0;
+    for (int k = 0; k < A.nd; k++) {
+      const double v = wave_sum_fixed(acc[k]);
+      if (lane == 0) A.part[(size_t)t * A.nd + k] = w * v;
+    }
+  }
+}
+
+constexpr int kConThreads = 256;
+
+__global__ __launch_bounds__(kConThreads) void k_con_reduce(ConArgs A) {
+  __shared__ double red[3][kConThreads];
+  __shared__ int fmin[kConThreads];
+  const int u = blockIdx.x, tid = threadIdx.x;
+  const int m = A.umol[u], g = A.uatom[u];
+  const int t0 = A.tmb[m], t1 = A.tmb[m + 1];
+  double s[3] = {0.0, 0.0, 0.0};
+  int first = INT_MAX;
+  for (int t = t0 + tid; t < t1; t += kConThreads) {
+    if (A.stat[2 * t]) first = min(first, 2 * t);
+    else if (A.stat[2 * t + 1]) first = min(first, 2 * t + 1);
+    if (A.nd == 1) {
+      s[0] += A.part[t];
+      continue;
+    }
+    const ecp_triplet tr = A.trip[t];
+    const int X = A.shells[tr.shifta != 0 ? tr.sa : tr.sb].atom, C = A.cent[tr.centre].atom;
+    const double *v = A.part + (size_t)t * 3;
+    if (X == g)
+      for (int k = 0; k < 3; k++) s[k] += v[k];
+    if (C == g)
+      for (int k = 0; k < 3; k++) s[k] -= v[k];
+  }
+  for (int k = 0; k < 3; k++) red[k][tid] = s[k];
+  fmin[tid] = first;
+  __syncthreads();
+  for (int w = kConThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      for (int k = 0; k < 3; k++) red[k][tid] += red[k][tid + w];
+      fmin[tid] = min(fmin[tid], fmin[tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid < A.nd) A.out[A.uout[u] + tid] = red[tid][0];
+  // one unit per molecule reports its first failure (callback order)
+  if (tid == 0 && (u == 0 || A.umol[u - 1] != m))
+    A.codes[m] = fmin[0] == INT_MAX ? 0 : 1 + (fmin[0] & 1);
+}
+}  // namespace
+
+extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P, size_t nP, int P_on_device,
+                                const long long *poff, const int *prow, double *out, int out_on_device,
+                                int *codes, double *ms, char *err, size_t errlen) {
+  const Prepared &Pp = d->P;
+  if (!d->prepared) {
+    snprintf(err, errlen, "no evaluated job to contract");
+    return -1;
+  }
+  if (job->deriv > 1) {
+    snprintf(err, errlen, "density contraction supports derivative orders 0 and 1");
+    return -1;
+  }
+  const int nmol = job->nmol, nd = job->deriv == 0 ? 1 : 3;
+  // output units: one per molecule (energy) or per atom (gradient)
+  std::vector<int> umol, uatom, uout, aoff((size_t)job->nsh + 1);
+  ecp_job_ao_offsets(job, aoff.data());
+  size_t nout = 0;
+  if (nd == 1) {
+    for (int m = 0; m < nmol; m++) {
+      umol.push_back(m);
+      uatom.push_back(-1);
+      uout.push_back((int)nout++);
+    }
+  } else {
+    for (int m = 0; m < nmol; m++) {
+      for (int g = job->mol_atom0[m]; g < job->mol_atom0[m + 1]; g++) {
+        umol.push_back(m);
+        uatom.push_back(g);
+        uout.push_back((int)nout);
+        nout += 3;
+      }
+    }
+  }
+  const int nunits = (int)umol.size();
+  // device layout
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t o = 0;
+  const size_t o_part = o; o += al(8 * (size_t)std::max(1, Pp.nt) * nd);
+  const size_t o_out = o; o += al(8 * std::max<size_t>(1, nout));
+  const size_t o_codes = o; o += al(4 * (size_t)nmol);
+  const size_t o_idx = o;
+  const size_t n_idx = 8 * (size_t)nmol + 4 * ((size_t)job->nsh + 2 * (size_t)nmol + 1 + 3 * (size_t)nunits);
+  o += al(n_idx);
+  const size_t o_P = o; o += P_on_device ? 0 : al(8 * nP);
+  if (o > d->con_cap) {
+    if (d->con_mem) ECP_CHECK(hipFree(d->con_mem));
+    d->con_mem = nullptr;
+    d->con_cap = 0;
+    ECP_CHECK(hipMalloc(&d->con_mem, o));
+    d->con_cap = o;
+  }
+  if (!d->con_ev[0]) {
+    ECP_CHECK(hipEventCreate(&d->con_ev[0]));
+    ECP_CHECK(hipEventCreate(&d->con_ev[1]));
+  }
+  std::vector<char> h(n_idx);
+  size_t q = 0;
+  auto put = [&](const void *src, size_t bytes) {
+    memcpy(h.data() + q, src, bytes);
+    q += bytes;
+    return o_idx + q - bytes;
+  };
+  ConArgs A{};
+  char *b = d->con_mem;
+  const size_t i_poff = put(poff, 8 * (size_t)nmol);
+  const size_t i_aoff = put(aoff.data(), 4 * (size_t)job->nsh);
+  const size_t i_prow = put(prow, 4 * (size_t)nmol);
+  const size_t i_tmb = put(job->trip_mol_begin, 4 * ((size_t)nmol + 1));
+  const size_t i_umol = put(umol.data(), 4 * (size_t)nunits);
+  const size_t i_uatom = put(uatom.data(), 4 * (size_t)nunits);
+  const size_t i_uout = put(uout.data(), 4 * (size_t)nunits);
+  ECP_CHECK(hipMemcpyAsync(b + o_idx, h.data(), q, hipMemcpyHostToDevice, d->stream));
+  if (!P_on_device && nP) ECP_CHECK(hipMemcpyAsync(b + o_P, P, 8 * nP, hipMemcpyHostToDevice, d->stream));
+  A.trip = Pp.J.trip;
+  A.shells = Pp.J.shells;
+  A.cent = Pp.J.cent;
+  A.blocks = Pp.blk;
+  A.stat = Pp.stat;
+  A.aoff = (const int *)(b + i_aoff);
+  A.P = P_on_device ? P : (const double *)(b + o_P);
+  A.poff = (const long long *)(b + i_poff);
+  A.prow = (const int *)(b + i_prow);
+  A.tmb = (const int *)(b + i_tmb);
+  A.umol = (const int *)(b + i_umol);
+  A.uatom = (const int *)(b + i_uatom);
+  A.uout = (const int *)(b + i_uout);
+  A.part = (double *)(b + o_part);
+  A.out = out_on_device ? out : (double *)(b + o_out);
+  A.codes = (int *)(b + o_codes);
+  A.ntrip = Pp.nt;
+  A.nd = nd;
+  A.nunits = nunits;
+  A.nmol = nmol;
+  ECP_CHECK(hipEventRecord(d->con_ev[0], d->stream));
+  if (Pp.nt > 0) {
+    k_con_part<<<std::min(Pp.nt, 65536), 64, 0, d->stream>>>(d->T, A);
+    ECP_CHECK(hipGetLastError());
+  }
+  if (nunits > 0) {
+    k_con_reduce<<<nunits, kConThreads, 0, d->stream>>>(A);
+    ECP_CHECK(hipGetLastError());
+  }
+  ECP_CHECK(hipEventRecord(d->con_ev[1], d->stream));
+  if (!out_on_device && nout)
+    ECP_CHECK(hipMemcpyAsync(out, b + o_out, 8 * nout, hipMemcpyDeviceToHost, d->stream));
+  ECP_CHECK(hipMemcpyAsync(codes, b + o_codes, 4 * (size_t)nmol, hipMemcpyDeviceToHost, d->stream));
+  ECP_CHECK(hipStreamSynchronize(d->stream));
+  if (ms) {
+    float f = 0.f;
+    ECP_CHECK(hipEventElapsedTime(&f, d->con_ev[0], d->con_ev[1]));
+    *ms = f;
+  }
+  return 0;
 }
 
 extern "C" int ecp_dev_mem_alloc(void **p, size_t bytes, char *err, size_t errlen) {

@@ -59,6 +59,7 @@ typedef struct {
   int ntrip;
   ecp_triplet *trip;         /* [ntrip], per molecule in the reference order */
   int *trip_mol_begin;       /* [nmol+1] */
+  int *mol_atom0;            /* [nmol+1] first global atom of each molecule */
   size_t nblk;               /* doubles of output blocks */
   int maxK;                  /* largest contraction length */
   int maxL;                  /* largest ECP local label */
@@ -107,6 +108,16 @@ int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char *err, size_
 int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t errlen);
 int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *err, size_t errlen);
 int ecp_dev_fetch(ecp_dev *d, double *blocks, unsigned char *status, char *err, size_t errlen);
+/* Contracts the last evaluation's blocks with densities on the device (job
+   of derivative order 0: one energy per molecule; order 1: 3 gradient
+   components per atom, molecule after molecule).  P holds nP doubles,
+   molecule m's symmetric density at poff[m] with row stride prow[m]; P and
+   out are device pointers when the *_on_device flags are set.  codes[m] =
+   0, or 1 / 2 when a type-1 / type-2 integration of molecule m failed.
+   *ms = device time of the contraction kernels.  0, or -1 (message in err). */
+int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P, size_t nP, int P_on_device,
+                     const long long *poff, const int *prow, double *out, int out_on_device, int *codes,
+                     double *ms, char *err, size_t errlen);
 /* One eager evaluation timed per kernel launch (HIP events on the launch's
    own stream).  Per kernel family (tables, type1, type2, fallback1,
    fallback2, assemble): launches, summed duration in ms, algorithmic FLOPs

@@ -209,6 +209,7 @@ void ecp_job_free(ecp_job *job) {
   free(job->win);
   free(job->trip);
   free(job->trip_mol_begin);
+  free(job->mol_atom0);
   memset(job, 0, sizeof(*job));
 }
 
@@ -296,6 +297,7 @@ int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st
   job->gauss = calloc((size_t)ngauss + 1, sizeof(ecp_gauss));
   job->win = calloc((size_t)nwin + 1, sizeof(ecp_window));
   job->trip_mol_begin = calloc((size_t)nmol + 1, sizeof(int));
+  job->mol_atom0 = calloc((size_t)nmol + 1, sizeof(int));
   double *radius = calloc((size_t)nsh + 1, sizeof(double));
   trip_vec tv = {0};
   int a0 = 0, s0 = 0, p0 = 0, c0 = 0, g0 = 0, w0 = 0;
@@ -356,6 +358,7 @@ int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st
       cc++;
     }
     job->trip_mol_begin[m] = tv.n;
+    job->mol_atom0[m] = a0;
     /* per centre: potential cut-offs, windows, triplets */
     for (int c = c0; c < cc && !rc; c++) {
       ecp_centre *Cn = &job->cent[c];
@@ -464,6 +467,7 @@ int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st
     w0 += dims[m].necp * dims[m].nsh;
   }
   job->trip_mol_begin[nmol] = tv.n;
+  job->mol_atom0[nmol] = a0;
   job->trip = tv.v;
   job->ntrip = tv.n;
   job->nblk = blk;

@@ -94,6 +94,18 @@ class Handle:
         rc = lib().calculateECPIntegrals(C.c_void_p(self.h), f, None)
         return rc, keys, blocks
 
+    def contract(self, P):
+        """libecp_contract: the energy (n = 0, a float) or the gradient
+        (n = 1, an array of shape (nat, 3)) of the density P on the device.
+        Returns (rc, value)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        out = np.zeros(1 if self.n == 0 else 3 * self._m.nat)
+        rc = lib().libecp_contract(C.c_void_p(self.h), C.c_int(P.shape[1]), _P(P, C.c_double),
+                                   _P(out, C.c_double))
+        if rc < 0:
+            raise RuntimeError("libecp_contract failed: " + last_error())
+        return rc, (float(out[0]) if self.n == 0 else out.reshape(-1, 3))
+
     def close(self):
         if self.h:
             lib().libECP_free(C.c_void_p(self.h))
@@ -124,16 +136,58 @@ def _mol_struct(m):
 class Batch:
     """libecp_batch_*: many molecules in one resident device job."""
 
-    def __init__(self, mols, grid=1024, tol=1e-12, acc=1e-14):
+    def __init__(self, mols, grid=1024, tol=1e-12, acc=1e-14, n=0):
         L = lib()
-        L.libecp_batch_create.restype = C.c_void_p
+        L.libecp_batch_create_n.restype = C.c_void_p
         L.libecp_batch_triplets.restype = C.c_longlong
+        L.libecp_batch_contract.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         self.mols = list(mols)
+        self.n = n
         arr = (_Mol * len(self.mols))(*[_mol_struct(m) for m in self.mols])
-        self.b = L.libecp_batch_create(arr, C.c_int(len(self.mols)), C.c_int(grid), C.c_double(tol),
-                                       C.c_double(acc))
+        self.b = L.libecp_batch_create_n(arr, C.c_int(len(self.mols)), C.c_int(n), C.c_int(grid),
+                                         C.c_double(tol), C.c_double(acc))
         if not self.b:
             raise RuntimeError("libecp_batch_create failed: " + last_error())
+
+    def contract_sizes(self):
+        """(doubles of the packed densities, doubles of the packed output)."""
+        nP = sum(m.nao * m.nao for m in self.mols)
+        nout = len(self.mols) if self.n == 0 else sum(3 * m.nat for m in self.mols)
+        return nP, nout
+
+    def contract(self, Ps):
+        """libecp_batch_contract with host arrays: Ps is one density per
+        molecule.  Returns (codes, outputs, ms): energies (n = 0) or (nat, 3)
+        gradients (n = 1) per molecule."""
+        P = np.concatenate([np.ascontiguousarray(p, dtype=np.float64).ravel() for p in Ps])
+        nP, nout = self.contract_sizes()
+        assert P.size == nP
+        out = np.zeros(nout)
+        codes = np.zeros(len(self.mols), dtype=np.int32)
+        ms = C.c_double()
+        if lib().libecp_batch_contract(C.c_void_p(self.b), P.ctypes.data, out.ctypes.data, codes.ctypes.data,
+                                       0, C.addressof(ms)) != 0:
+            raise RuntimeError("libecp_batch_contract failed: " + last_error())
+        return codes, self._split(out), ms.value
+
+    def contract_device(self, P_ptr, out_ptr):
+        """libecp_batch_contract with device pointers (packed densities in,
+        packed results out).  Returns (codes, ms)."""
+        codes = np.zeros(len(self.mols), dtype=np.int32)
+        ms = C.c_double()
+        if lib().libecp_batch_contract(C.c_void_p(self.b), C.c_void_p(P_ptr), C.c_void_p(out_ptr),
+                                       codes.ctypes.data, 1, C.addressof(ms)) != 0:
+            raise RuntimeError("libecp_batch_contract failed: " + last_error())
+        return codes, ms.value
+
+    def _split(self, out):
+        if self.n == 0:
+            return [float(v) for v in out]
+        res, o = [], 0
+        for m in self.mols:
+            res.append(out[o:o + 3 * m.nat].reshape(-1, 3).copy())
+            o += 3 * m.nat
+        return res
 
     def execute(self, repeats=1):
         ms = C.c_double()
