@@ -19,6 +19,7 @@ FP64 peak, and the reference C library on the host's cores.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import multiprocessing as mp
 import os
@@ -36,6 +37,7 @@ from libecp_amd import native  # noqa: E402
 from libecp_amd.workloads import medium_cluster, serving_requests, uneven_batch  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBPS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_medium_cluster.md")
 
 
@@ -207,6 +209,81 @@ def serve_bench(n_req, slots, budget_gib, max_batch, load=0.7, seed=0):
     }
 
 
+def contract_bench(m, reps=10):
+    """Energy and gradient of one density for the medium cluster, contracted
+    on the device (libecp_batch_contract), against the host route a caller
+    of the reference takes (blocks to the host, matrix, trace)."""
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(m.nao, m.nao // 2))
+    P = X @ X.T / m.nao
+    res = {}
+    for n in (0, 1):
+        b = native.Batch([m], n=n)
+        nominal, evaluated = b.triplets()
+        b.execute(2)
+        b.contract([P])
+        t = time.perf_counter()
+        for _ in range(reps):
+            b.execute(1)
+            codes, outs, kms = b.contract([P])
+        w = (time.perf_counter() - t) / reps
+        kk = []
+        for _ in range(reps):
+            kk.append(b.contract([P])[2])
+        # the density already resident in HBM (a GPU-side SCF): device buffers
+        # from the HIP runtime the library runs on
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        nP, nout = b.contract_sizes()
+        Pd, Od = ctypes.c_void_p(), ctypes.c_void_p()
+        Pc = np.ascontiguousarray(P)
+        if hip.hipMalloc(ctypes.byref(Pd), ctypes.c_size_t(8 * nP)) or \
+                hip.hipMalloc(ctypes.byref(Od), ctypes.c_size_t(8 * nout)) or \
+                hip.hipMemcpy(Pd, ctypes.c_void_p(Pc.ctypes.data), ctypes.c_size_t(8 * nP), 1):
+            raise RuntimeError("device buffers for the resident-density run")
+        b.contract_device(Pd.value, Od.value)
+        t = time.perf_counter()
+        for _ in range(reps):
+            b.execute(1)
+            b.contract_device(Pd.value, Od.value)
+        w_res = (time.perf_counter() - t) / reps
+        hip.hipFree(Pd)
+        hip.hipFree(Od)
+        t = time.perf_counter()
+        b.fetch()
+        w_fetch = time.perf_counter() - t
+        r = {"ms_per_step": 1e3 * w, "value": nominal / w, "unit": "shell-triplets/s",
+             "resident_density_ms_per_step": 1e3 * w_res,
+             "contract_kernels_ms": statistics.median(kk), "blocks_fetch_ms": 1e3 * w_fetch,
+             "evaluated_triplets": evaluated, "status": int(codes[0])}
+        if n == 0:
+            r["energy"] = outs[0]
+            t = time.perf_counter()
+            for _ in range(reps):
+                b.execute(1)
+                b.fetch()
+                rc, I = b.matrix(0)
+                V = I + I.T - np.diag(np.diag(I))
+                e_host = float(np.sum(P * V))
+            w_host = (time.perf_counter() - t) / reps
+            r["host_route_ms_per_step"] = 1e3 * w_host
+            r["host_route"] = "evaluate, blocks to the host, getIntegrals matrix, numpy trace"
+            r["vs_host_route"] = w_host / w
+            r["resident_density_vs_host_route"] = w_host / w_res
+            r["abs_diff_vs_host_route"] = abs(e_host - outs[0])
+        else:
+            r["gradient_norm"] = float(np.linalg.norm(outs[0]))
+            r["host_route_lower_bound"] = "blocks_fetch_ms: the device-to-host copy alone"
+        # algorithmic HBM bytes of the contraction: every block element once,
+        # one density element per (type-1, type-2) element pair, the partials
+        nblk = b.contract_block_doubles()
+        byts = 8 * nblk + 8 * nblk // 2 + 8 * evaluated * (1 if n == 0 else 3)
+        r["contract_gbps"] = byts / (r["contract_kernels_ms"] * 1e-3) / 1e9
+        r["contract_hbm_frac"] = r["contract_gbps"] / HBM_PEAK_GBPS
+        b.close()
+        res["energy" if n == 0 else "gradient"] = r
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +293,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-contract", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="run the launch-parameter autotuner on each job before timing (default: built-in choices)")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -351,6 +429,9 @@ def main():
                                      args.serve_max_batch)
         if "plain" in out:
             out["serving"]["vs_plain"] = out["serving"]["value"] / out["plain"]["value"]
+
+    if not args.no_contract:
+        out["contraction"] = contract_bench(m)
 
     if not args.no_cpu:
         cpu, I_ref = cpu_baseline(m, nominal)

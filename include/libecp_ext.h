@@ -84,6 +84,9 @@ int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I);
 int libecp_batch_contract(libecp_batch *b, const double *P, double *out, int *codes, int device, double *ms);
 /* Nominal (pre-screening) and evaluated triplet counts of the batch. */
 long long libecp_batch_triplets(libecp_batch *b, long long *evaluated);
+/* Doubles of the evaluated blocks (type 1 and type 2) the batch holds on
+   the device. */
+size_t libecp_batch_block_doubles(libecp_batch *b);
 /* Stage timings (ms) of the first evaluation: tables, quadratures (small
    grid and large-grid fallbacks), assembly; returns the count written. */
 int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n);

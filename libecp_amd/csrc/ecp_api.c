@@ -454,6 +454,8 @@ int libecp_batch_contract(libecp_batch *b, const double *P, double *out, int *co
   return rc;
 }
 
+size_t libecp_batch_block_doubles(libecp_batch *b) { return b ? b->job.nblk : 0; }
+
 long long libecp_batch_triplets(libecp_batch *b, long long *evaluated) {
   if (!b) return -1;
   if (evaluated) *evaluated = b->job.ntrip;

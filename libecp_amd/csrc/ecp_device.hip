@@ -1918,6 +1918,8 @@ struct ecp_dev {
   char *con_mem = nullptr;
   size_t con_cap = 0;
   hipEvent_t con_ev[2] = {};
+  std::vector<char> con_idx;  // index tables last uploaded to con_mem at con_idx_at
+  size_t con_idx_at = 0;
 };
 
 namespace {
@@ -3054,6 +3056,7 @@ extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P,
     d->con_cap = 0;
     ECP_CHECK(hipMalloc(&d->con_mem, o));
     d->con_cap = o;
+    d->con_idx.clear();
   }
   if (!d->con_ev[0]) {
     ECP_CHECK(hipEventCreate(&d->con_ev[0]));
@@ -3075,7 +3078,13 @@ extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P,
   const size_t i_umol = put(umol.data(), 4 * (size_t)nunits);
   const size_t i_uatom = put(uatom.data(), 4 * (size_t)nunits);
   const size_t i_uout = put(uout.data(), 4 * (size_t)nunits);
-  ECP_CHECK(hipMemcpyAsync(b + o_idx, h.data(), q, hipMemcpyHostToDevice, d->stream));
+  // the index tables change with the job or the density layout only
+  if (d->con_idx_at != o_idx || d->con_idx.size() != q || memcmp(d->con_idx.data(), h.data(), q) != 0) {
+    ECP_CHECK(hipMemcpyAsync(b + o_idx, h.data(), q, hipMemcpyHostToDevice, d->stream));
+    ECP_CHECK(hipStreamSynchronize(d->stream));
+    d->con_idx.assign(h.begin(), h.begin() + q);
+    d->con_idx_at = o_idx;
+  }
   if (!P_on_device && nP) ECP_CHECK(hipMemcpyAsync(b + o_P, P, 8 * nP, hipMemcpyHostToDevice, d->stream));
   A.trip = Pp.J.trip;
   A.shells = Pp.J.shells;

@@ -149,6 +149,10 @@ class Batch:
         if not self.b:
             raise RuntimeError("libecp_batch_create failed: " + last_error())
 
+    def contract_block_doubles(self):
+        lib().libecp_batch_block_doubles.restype = C.c_size_t
+        return int(lib().libecp_batch_block_doubles(C.c_void_p(self.b)))
+
     def contract_sizes(self):
         """(doubles of the packed densities, doubles of the packed output)."""
         nP = sum(m.nao * m.nao for m in self.mols)
