@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libecp_amd.so")
 
-C_SOURCES = ["ecp_tables.c", "ecp_job.c", "ecp_api.c", "ecp_serve.c"]
+C_SOURCES = ["ecp_tables.c", "ecp_job.c", "ecp_api.c", "ecp_serve.c", "ecp_io.c"]
 HIP_SOURCES = ["ecp_device.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -61,6 +61,7 @@ ARCHIVE = os.path.join(PKG, "libecp_amd.a")
 EXAMPLE_SRC = os.path.join(ROOT, "tests", "c", "example_getintegrals.c")
 EXAMPLES = {"shared": os.path.join(ROOT, "tests", "c", "example_getintegrals"),
             "static": os.path.join(ROOT, "tests", "c", "example_getintegrals_static")}
+CLI = os.path.join(PKG, "ecp_integrals")
 
 
 def build_examples() -> None:
@@ -79,6 +80,10 @@ def build_examples() -> None:
           "-Wl,-rpath,$ORIGIN/../../libecp_amd", "-lm"])
     _run([HIPCC, "--hip-link", "--offload-arch=" + ARCH, eo, ARCHIVE, "-o", EXAMPLES["static"], "-lm",
           "-lpthread"])
+    # the command-line program (file inputs of the reference's example)
+    co = os.path.join(BUILD, "ecp_cli.o")
+    _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "ecp_cli.c"), "-o", co])
+    _run(["gcc", co, "-o", CLI, "-L" + PKG, "-lecp_amd", "-Wl,-rpath,$ORIGIN", "-lm"])
 
 
 def build_oracle(with_reference: bool = True) -> None:

@@ -59,6 +59,63 @@ def get_integrals(m, grid=1024, tol=1e-12, acc=1e-14, I=None):
     return rc, I
 
 
+def _take(ptr, n, ctype, dtype):
+    """Copies n values from a malloc'ed C array and frees it."""
+    a = np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n].astype(dtype, copy=True)
+    _libc().free(ptr)
+    return a
+
+
+_libc_h = None
+
+
+def _libc():
+    global _libc_h
+    if _libc_h is None:
+        _libc_h = C.CDLL(None)
+        _libc_h.free.argtypes = [C.c_void_p]
+    return _libc_h
+
+
+def load_molecule(xyz, ecp, basis, name=""):
+    """libecp_load_xyz / libecp_load_ecp / libecp_load_basis: the three input
+    files of the reference's example program as a Molecule."""
+    from .workloads import Molecule
+    L = lib()
+    nat = C.c_int()
+    g = C.POINTER(C.c_double)()
+    if L.libecp_load_xyz(xyz.encode(), C.byref(nat), C.byref(g)) < 0:
+        raise RuntimeError(last_error())
+    n = nat.value
+    geometry = _take(g, 3 * n, C.c_double, np.float64)
+    ip = [C.POINTER(C.c_int)() for _ in range(4)]
+    dp = [C.POINTER(C.c_double)() for _ in range(3)]
+    ne = L.libecp_load_ecp(ecp.encode(), nat, C.byref(ip[0]), C.byref(ip[1]), C.byref(ip[2]), C.byref(dp[0]),
+                           C.byref(dp[1]), C.byref(dp[2]), C.byref(ip[3]))
+    if ne < 0:
+        raise RuntimeError(last_error())
+    shellsECP = _take(ip[0], n, C.c_int, np.int32)
+    lECP = _take(ip[1], ne, C.c_int, np.int32)
+    KECP = _take(ip[2], ne, C.c_int, np.int32)
+    npe = int(KECP.sum())
+    aECP, dECP, nECP = (_take(p, npe, C.c_double, np.float64) for p in dp)
+    nCore = _take(ip[3], n, C.c_int, np.int32)
+    ib = [C.POINTER(C.c_int)() for _ in range(3)]
+    db = [C.POINTER(C.c_double)() for _ in range(2)]
+    nb = L.libecp_load_basis(basis.encode(), nat, C.byref(ib[0]), C.byref(ib[1]), C.byref(ib[2]),
+                             C.byref(db[0]), C.byref(db[1]))
+    if nb < 0:
+        raise RuntimeError(last_error())
+    shellsBS = _take(ib[0], n, C.c_int, np.int32)
+    lBS = _take(ib[1], nb, C.c_int, np.int32)
+    KBS = _take(ib[2], nb, C.c_int, np.int32)
+    npb = int(KBS.sum())
+    aBS, dBS = (_take(p, npb, C.c_double, np.float64) for p in db)
+    return Molecule(geometry=geometry, shellsECP=shellsECP, lECP=lECP, KECP=KECP, nECP=nECP, dECP=dECP,
+                    aECP=aECP, shellsBS=shellsBS, lBS=lBS, KBS=KBS, dBS=dBS, aBS=aBS, name=name,
+                    meta={"nCore": nCore})
+
+
 class Handle:
     """libECP_init / calculateECPIntegrals / libECP_free."""
 
