@@ -318,21 +318,27 @@ __device__ __forceinline__ double dist(const ecp_xyz &a, const ecp_xyz &b) {
 // ---------------------------------------------------------------------------
 // per-centre potential tables
 
-__global__ __launch_bounds__(384) void k_utab(DevTables T, DevJob J) {
-  const int c = blockIdx.x, n = threadIdx.x;
-  if (n >= SO) return;
+// U tables of one (centre, row, chunk of 128 points): row C.L is the local
+// channel U_L (type 1), rows l < L the semi-local channels times r^k,
+// k < UK (type 2), zero past the channel's cut-off end[l].
+__device__ void utab_rows(const DevTables &T, const DevJob &J, int c, int row, int chunk) {
   const ecp_centre C = J.cent[c];
+  if (row > C.L) return;
+  const int n = chunk * 128 + (int)threadIdx.x;
+  if (n >= SO) return;
   const ecp_gauss *g = J.gauss + C.g0;
   const double r = T.xs[n];
-  J.UL[C.UL_off + n] = eval_ecp(g, C.ng, C.L, r);
-  for (int l = 0; l < C.L; l++) {
-    double P = (n <= C.end[l]) ? eval_ecp(g, C.ng, l, r) : 0.0;
-    double *dst = J.Ut + C.Ut_off + (size_t)l * J.UK * SO + n;
-    dst[0] = P;
-    for (int k = 1; k < J.UK; k++) {
-      P = P * r;
-      dst[(size_t)k * SO] = P;
-    }
+  if (row == C.L) {
+    J.UL[C.UL_off + n] = eval_ecp(g, C.ng, C.L, r);
+    return;
+  }
+  const int l = row;
+  double P = (n <= C.end[l]) ? eval_ecp(g, C.ng, l, r) : 0.0;
+  double *dst = J.Ut + C.Ut_off + (size_t)l * J.UK * SO + n;
+  dst[0] = P;
+  for (int k = 1; k < J.UK; k++) {
+    P = P * r;
+    dst[(size_t)k * SO] = P;
   }
 }
 
@@ -370,8 +376,7 @@ __device__ void ftab_slab(const DevTables &T, const DevJob &J, const ecp_centre 
   }
 }
 
-__global__ __launch_bounds__(128) void k_ftab(DevTables T, DevJob J, const int2 *__restrict__ items) {
-  const int2 it = items[blockIdx.x];
+__device__ void ftab_item(const DevTables &T, const DevJob &J, int2 it) {
   const int sl = it.y >> 2, run = it.y & 3;  // (shell, run) packed
   const ecp_centre C = J.cent[it.x];
   const ecp_window W = J.win[C.win_off + sl];
@@ -389,10 +394,10 @@ __global__ __launch_bounds__(128) void k_ftab(DevTables T, DevJob J, const int2 
 }
 
 // per-(centre, atom) angular table ang[lambda][LM(l,m)][C(la,c)]
-__global__ __launch_bounds__(128) void k_ang(DevTables T, DevJob J, const int2 *__restrict__ items,
-                                             const double *__restrict__ fac, const double *__restrict__ dfac) {
-  __shared__ double rsph[(ECP_MAX_L + ECP_MAX_LBS) * (ECP_MAX_L + ECP_MAX_LBS)];
-  const int2 it = items[blockIdx.x];
+constexpr int kAngRsph = (ECP_MAX_L + ECP_MAX_LBS) * (ECP_MAX_L + ECP_MAX_LBS);
+
+__device__ void ang_item(const DevTables &T, const DevJob &J, int2 it, const double *__restrict__ fac,
+                         const double *__restrict__ dfac, double *rsph) {
   const ecp_centre C = J.cent[it.x];
   const int A = C.atom0 + it.y;
   const int lmax_a = J.atom_maxl[A] + J.deriv;
@@ -1819,11 +1824,43 @@ struct DBuf {
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// zero-fill of per-run counters and flags (a kernel, so that the sequence
-// can be captured in a HIP graph without memset nodes)
-__global__ void k_zero(unsigned int *__restrict__ p, size_t nwords) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
-    p[i] = 0u;
+// The per-job tables in one launch of 128-thread blocks: the zero-fill of
+// per-run counters and flags (a kernel, so that the sequence can be captured
+// in a HIP graph without memset nodes), the U tables per (centre, row, point
+// chunk), the F slabs per (centre, shell, run) and the angular tables per
+// (centre, atom).  They are independent of each other; every consumer is a
+// later launch.
+struct TabArgs {
+  unsigned int *zero_base;
+  size_t zero_words;
+  int nzb, nub, nfit, nait, rows;  // rows: U-table rows per centre (largest L + 1)
+  const int2 *fit, *ait;
+  const double *fac, *dfac;
+};
+
+constexpr int kUtabChunks = (SO + 127) / 128;
+
+__global__ __launch_bounds__(128) void k_tables(DevTables T, DevJob J, TabArgs A) {
+  __shared__ double rsph[kAngRsph];
+  int b = blockIdx.x;
+  if (b < A.nzb) {
+    for (size_t i = (size_t)b * 128 + threadIdx.x; i < A.zero_words; i += (size_t)A.nzb * 128) A.zero_base[i] = 0u;
+    return;
+  }
+  b -= A.nzb;
+  if (b < A.nub) {
+    const int per_c = A.rows * kUtabChunks;
+    const int c = b / per_c, rem = b - c * per_c;
+    utab_rows(T, J, c, rem / kUtabChunks, rem % kUtabChunks);
+    return;
+  }
+  b -= A.nub;
+  if (b < A.nfit) {
+    ftab_item(T, J, A.fit[b]);
+    return;
+  }
+  b -= A.nfit;
+  if (b < A.nait) ang_item(T, J, A.ait[b], A.fac, A.dfac, rsph);
 }
 
 // byte offsets of a job's arrays inside its device block: inputs (uploaded)
@@ -2239,6 +2276,23 @@ void launch_fb2(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, 
 // overlaps the quadratures).
 void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs);
 
+void launch_tables(ecp_dev *d, hipStream_t s) {
+  const Prepared &P = d->P;
+  TabArgs A{};
+  A.zero_base = P.zero_base;
+  A.zero_words = P.zero_words;
+  A.nzb = (int)std::max<size_t>(1, std::min<size_t>((P.zero_words + 127) / 128, 2048));
+  A.rows = P.max_L + 1;
+  A.nub = P.ncent * A.rows * kUtabChunks;
+  A.nfit = P.nfit;
+  A.nait = P.nait;
+  A.fit = P.fit;
+  A.ait = P.ait;
+  A.fac = d->fac_dev;
+  A.dfac = d->dfac_dev;
+  k_tables<<<A.nzb + A.nub + A.nfit + A.nait, 128, 0, s>>>(d->T, P.J, A);
+}
+
 // fuse = 2 (automatic): fused launches for jobs of up to this many type-1
 // primitive pairs (latency-bound: per-class launches hold few waves each)
 constexpr int kFuseAutoMaxUnits = 150000;
@@ -2256,10 +2310,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (evs) (void)hipEventRecord(evs[0], s);
   {
     Span sp(d, s, FAM_TABLES);
-    k_zero<<<(int)std::min<size_t>((P.zero_words + 255) / 256, 1024), 256, 0, s>>>(P.zero_base, P.zero_words);
-    k_utab<<<P.ncent, 384, 0, s>>>(T, J);
-    if (P.nfit) k_ftab<<<P.nfit, 128, 0, s>>>(T, J, P.fit);
-    if (P.nait) k_ang<<<P.nait, 128, 0, s>>>(T, J, P.ait, d->fac_dev, d->dfac_dev);
+    launch_tables(d, s);
   }
   if (evs) (void)hipEventRecord(evs[1], s);
   (void)hipEventRecord(d->fork, s);
@@ -2345,10 +2396,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (evs) (void)hipEventRecord(evs[0], s);
   {
     Span sp(d, s, FAM_TABLES);
-    k_zero<<<(int)std::min<size_t>((P.zero_words + 255) / 256, 1024), 256, 0, s>>>(P.zero_base, P.zero_words);
-    k_utab<<<P.ncent, 384, 0, s>>>(T, J);
-    if (P.nfit) k_ftab<<<P.nfit, 128, 0, s>>>(T, J, P.fit);
-    if (P.nait) k_ang<<<P.nait, 128, 0, s>>>(T, J, P.ait, d->fac_dev, d->dfac_dev);
+    launch_tables(d, s);
   }
   if (evs) (void)hipEventRecord(evs[1], s);
   (void)hipEventRecord(d->fork, s);
