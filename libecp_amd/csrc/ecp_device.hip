@@ -2,12 +2,13 @@
 //
 // Pipeline of one job (all FP64; a main stream forks into worker streams
 // and joins them, captured once into a HIP graph):
-//   k_utab   U_L(r_n) and r_n^k U_l(r_n) on the 383-point small grid, per centre
-//            (evalECP / tabECP_FM06: ecp.c:41-60, type2.c:184-219)
-//   k_ftab   F_lambda(r_n) per (centre, shell) on its screened window
+//   k_tables zero-fill of counters and flags; U_L(r_n) and r_n^k U_l(r_n) on
+//            the 383-point small grid per (centre, channel, point chunk)
+//            (evalECP / tabECP_FM06: ecp.c:41-60, type2.c:184-219); and
+//            F_lambda(r_n) per (centre, shell) on its screened window
 //            (calcF_FM06: type2.c:222-310)
 //   k_ang    per-(centre, atom) angular tables (evalAngularIntegrals:
-//            angular_integrals.c:104-142)
+//            angular_integrals.c:104-142), concurrent with the quadratures
 //   k_type1 / k_t1tail  type-1 radial integrals Q[l1][l2] per (triplet,
 //            primitive pair) with the PS93 sequence (calcQ: type1.c:94-203):
 //            lane-serial head, wave-cooperative tail for the open pairs
@@ -851,14 +852,16 @@ __device__ __forceinline__ int psm92_round_point(const DevTables &T, int l0, int
 // each lane owning a failed entry sums them in the integrator's pair order
 // (gc_integrators.c:55-84) and applies the PSM92 test; the wave stops when
 // every entry has converged.  Points past the screening break contribute 0.
-constexpr int kKS = 65;  // LDS stride of per-point Bessel rows (bank spread)
+constexpr int kKS = 65;     // LDS stride of per-point Bessel rows (bank spread)
+constexpr int kFb1TS = 33;  // LDS stride of a failed entry's pair sums (at most 32 pairs per call)
 
 template <int LAB>
 __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, const DevJob &J,
                                          const T1Unit *__restrict__ units, double *__restrict__ qout,
                                          const unsigned long long *__restrict__ qfail, const int *__restrict__ fbq,
                                          const int *__restrict__ fbcount, unsigned char *__restrict__ status,
-                                         double *s_r, double *s_w, double *s_u, double *s_g, double *s_k) {
+                                         double *s_r, double *s_w, double *s_u, double *s_g, double *s_k,
+                                         double *s_t, int *s_e) {
   const int count = *fbcount;
   const int lane = threadIdx.x;
   for (int it = bid; it < count; it += nb) {
@@ -878,6 +881,7 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
     const int end = brk - 1;
     // this lane's failed entry (the lane-th set bit of mask)
     int my_e = -1, my_l1 = 0, my_l2 = 0;
+    const int nfail = __popcll(mask);
     {
       int e = 0, seen = 0;
       for (int l1 = 0; l1 <= LAB; l1++)
@@ -914,12 +918,14 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
 #pragma unroll
       for (int l = 0; l <= LAB; l++) s_k[l * kKS + slot] = K[l];
     };
-    auto f = [&](int slot) {
+    auto fj = [&](int l1, int l2, int slot) {
       double rn = 1.0;
-      for (int k = 0; k < my_l1; k++) rn = s_r[slot] * rn;
-      return s_w[slot] * ((((Cc * rn) * s_u[slot]) * s_k[my_l2 * kKS + slot]) * s_g[slot]);
+      for (int k = 0; k < l1; k++) rn = s_r[slot] * rn;
+      return s_w[slot] * ((((Cc * rn) * s_u[slot]) * s_k[l2 * kKS + slot]) * s_g[slot]);
     };
+    auto f = [&](int slot) { return fj(my_l1, my_l2, slot); };
     const bool mine = my_e >= 0;
+    if (mine) s_e[lane] = my_l1 | (my_l2 << 8);
     double I = 0.0, p = 0.0, res = 0.0;
     bool done = !mine;
     auto test = [&](int l, double q) {
@@ -930,14 +936,25 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
         res = 16 * I / (3 * N);
       }
     };
-    // pairs [kb, ke) of level l, pair k at slot0 + 2 (k - kb)
+    // pairs [kb, ke) of level l, pair k at slot0 + 2 (k - kb): the pair sums
+    // (0 + f(left)) + f(right) of every open entry by all lanes into s_t,
+    // then each entry's lane adds its own in pair order.  Called by the
+    // whole wave.
     auto sum = [&](int l, int slot0, int kb, int ke) {
-      const int b0 = T.s92.lvl_begin[l];
-      for (int k = kb; k < ke; k++) {
-        double t = f(slot0 + 2 * (k - kb));
-        if (T.p92r[b0 + k] <= end) t += f(slot0 + 2 * (k - kb) + 1);
-        I += t;
+      const int b0 = T.s92.lvl_begin[l], np = ke - kb;
+      const unsigned long long open = __ballot(mine && !done);
+      for (int i = lane; i < nfail * np; i += 64) {
+        const int j = i / np, k = i - j * np;
+        if (!((open >> j) & 1ull)) continue;
+        const int code = s_e[j];
+        double t = fj(code & 0xff, code >> 8, slot0 + 2 * k);
+        if (T.p92r[b0 + kb + k] <= end) t += fj(code & 0xff, code >> 8, slot0 + 2 * k + 1);
+        s_t[j * kFb1TS + k] = t;
       }
+      __syncthreads();
+      if (mine && !done)
+        for (int k = 0; k < np; k++) I += s_t[lane * kFb1TS + k];
+      __syncthreads();
     };
     int lv = 0;
     while (lv < T.s92.nlevels && wave_any(!done)) {
@@ -949,11 +966,14 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
         for (int l = lv; l < l1; l++) {
           if (l == 0) {
             if (mine) I = p = f(0);
-          } else if (!done) {
-            const double q = 2 * p;
-            p = 2 * I;
+          } else {
+            double q = 0.0;
+            if (!done) {
+              q = 2 * p;
+              p = 2 * I;
+            }
             sum(l, roff, 0, T.s92.lvl_begin[l + 1] - T.s92.lvl_begin[l]);
-            test(l, q);
+            if (!done) test(l, q);
           }
           roff += psm92_points(l);
         }
@@ -976,8 +996,7 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
           }
           eval(lane, n);
           __syncthreads();
-          if (!done) sum(lv, 0, c0 - b0, c0 - b0 + nc);
-          __syncthreads();
+          sum(lv, 0, c0 - b0, c0 - b0 + nc);
         }
         if (!done) test(lv, q);
         lv++;
@@ -996,8 +1015,10 @@ __global__ __launch_bounds__(64) void k_fb1(DevTables T, DevJob J, const T1Unit 
                                             double *__restrict__ qout, const unsigned long long *__restrict__ qfail,
                                             const int *__restrict__ fbq, const int *__restrict__ fbcount,
                                             unsigned char *__restrict__ status) {
-  __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(LAB + 1) * kKS];
-  fb1_body<LAB>(blockIdx.x, gridDim.x, T, J, units, qout, qfail, fbq, fbcount, status, s_r, s_w, s_u, s_g, s_k);
+  __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(LAB + 1) * kKS], s_t[nq_of(LAB) * kFb1TS];
+  __shared__ int s_e[64];
+  fb1_body<LAB>(blockIdx.x, gridDim.x, T, J, units, qout, qfail, fbq, fbcount, status, s_r, s_w, s_u, s_g, s_k,
+                s_t, s_e);
 }
 
 // ---------------------------------------------------------------------------
@@ -1479,7 +1500,8 @@ __global__ __launch_bounds__(64) void k_t1tail_fused(DevTables T, DevJob J, T1Se
 
 template <int MAXLAB>
 __global__ __launch_bounds__(64) void k_fb1_fused(DevTables T, DevJob J, T1Set S, ClassMap cm) {
-  __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(MAXLAB + 1) * kKS];
+  __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(MAXLAB + 1) * kKS], s_t[nq_of(MAXLAB) * kFb1TS];
+  __shared__ int s_e[64];
   const int i = class_entry(cm, blockIdx.x);
   const int c = cm.cls[i], bid = blockIdx.x - cm.bbeg[i], nb = cm.bbeg[i + 1] - cm.bbeg[i];
   const int ub = S.ubeg[c];
@@ -1488,7 +1510,7 @@ __global__ __launch_bounds__(64) void k_fb1_fused(DevTables T, DevJob J, T1Set S
   case L:                                                                                                      \
     if constexpr (L <= MAXLAB)                                                                                 \
       fb1_body<L>(bid, nb, T, J, S.units + ub, S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.stat, s_r, s_w, s_u, s_g, \
-                  s_k);                                                                                         \
+                  s_k, s_t, s_e);                                                                               \
     break;
     ECP_CASES_LAB(X)
 #undef X
@@ -1827,21 +1849,18 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 // The per-job tables in one launch of 128-thread blocks: the zero-fill of
 // per-run counters and flags (a kernel, so that the sequence can be captured
 // in a HIP graph without memset nodes), the U tables per (centre, row, point
-// chunk), the F slabs per (centre, shell, run) and the angular tables per
-// (centre, atom).  They are independent of each other; every consumer is a
-// later launch.
+// chunk) and the F slabs per (centre, shell, run).  They are independent of
+// each other; every consumer is a later launch.
 struct TabArgs {
   unsigned int *zero_base;
   size_t zero_words;
-  int nzb, nub, nfit, nait, rows;  // rows: U-table rows per centre (largest L + 1)
-  const int2 *fit, *ait;
-  const double *fac, *dfac;
+  int nzb, nub, nfit, rows;  // rows: U-table rows per centre (largest L + 1)
+  const int2 *fit;
 };
 
 constexpr int kUtabChunks = (SO + 127) / 128;
 
-__global__ __launch_bounds__(128) void k_tables(DevTables T, DevJob J, TabArgs A) {
-  __shared__ double rsph[kAngRsph];
+__global__ __launch_bounds__(128, 4) void k_tables(DevTables T, DevJob J, TabArgs A) {
   int b = blockIdx.x;
   if (b < A.nzb) {
     for (size_t i = (size_t)b * 128 + threadIdx.x; i < A.zero_words; i += (size_t)A.nzb * 128) A.zero_base[i] = 0u;
@@ -1855,12 +1874,15 @@ __global__ __launch_bounds__(128) void k_tables(DevTables T, DevJob J, TabArgs A
     return;
   }
   b -= A.nub;
-  if (b < A.nfit) {
-    ftab_item(T, J, A.fit[b]);
-    return;
-  }
-  b -= A.nfit;
-  if (b < A.nait) ang_item(T, J, A.ait[b], A.fac, A.dfac, rsph);
+  if (b < A.nfit) ftab_item(T, J, A.fit[b]);
+}
+
+// angular tables: a launch of their own (few registers, high occupancy),
+// needed only by the type-2 assembly
+__global__ __launch_bounds__(128) void k_ang(DevTables T, DevJob J, const int2 *__restrict__ items,
+                                             const double *__restrict__ fac, const double *__restrict__ dfac) {
+  __shared__ double rsph[kAngRsph];
+  ang_item(T, J, items[blockIdx.x], fac, dfac, rsph);
 }
 
 // byte offsets of a job's arrays inside its device block: inputs (uploaded)
@@ -1925,7 +1947,7 @@ constexpr int kWorkers = 4;
 struct ecp_dev {
   hipStream_t stream = nullptr;
   hipStream_t workers[kWorkers] = {};
-  hipEvent_t fork = nullptr, t2done = nullptr, t1done = nullptr, join[kWorkers] = {};
+  hipEvent_t fork = nullptr, t2done = nullptr, t1done = nullptr, angdone = nullptr, join[kWorkers] = {};
   DevTables T{};
   void *static_mem = nullptr;
   char *arena = nullptr;
@@ -2054,6 +2076,7 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
   if ((e = hipEventCreateWithFlags(&d->fork, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->t2done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->t1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
+  if ((e = hipEventCreateWithFlags(&d->angdone, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   for (int i = 0; i < 8; i++)
     if ((e = hipEventCreate(&d->ev[i])) != hipSuccess) return fail("hipEventCreate", e);
   build_templates(d);
@@ -2158,6 +2181,7 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   if (d->fork) (void)hipEventDestroy(d->fork);
   if (d->t2done) (void)hipEventDestroy(d->t2done);
   if (d->t1done) (void)hipEventDestroy(d->t1done);
+  if (d->angdone) (void)hipEventDestroy(d->angdone);
   for (auto e : d->pev) (void)hipEventDestroy(e);
   if (d->con_mem) (void)hipFree(d->con_mem);
   for (int i = 0; i < 2; i++)
@@ -2285,12 +2309,16 @@ void launch_tables(ecp_dev *d, hipStream_t s) {
   A.rows = P.max_L + 1;
   A.nub = P.ncent * A.rows * kUtabChunks;
   A.nfit = P.nfit;
-  A.nait = P.nait;
   A.fit = P.fit;
-  A.ait = P.ait;
-  A.fac = d->fac_dev;
-  A.dfac = d->dfac_dev;
-  k_tables<<<A.nzb + A.nub + A.nfit + A.nait, 128, 0, s>>>(d->T, P.J, A);
+  k_tables<<<A.nzb + A.nub + A.nfit, 128, 0, s>>>(d->T, P.J, A);
+}
+
+// the angular tables on stream s, concurrent with the quadratures; the
+// type-2 assembly waits for d->angdone
+void launch_ang(ecp_dev *d, hipStream_t s) {
+  const Prepared &P = d->P;
+  if (P.nait) k_ang<<<P.nait, 128, 0, s>>>(d->T, P.J, P.ait, d->fac_dev, d->dfac_dev);
+  (void)hipEventRecord(d->angdone, s);
 }
 
 // fuse = 2 (automatic): fused launches for jobs of up to this many type-1
@@ -2319,9 +2347,14 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   // large-grid work there; the type-2 fallbacks and the assembly of the rest
   // on the main stream, so that neither waits behind type-1 work
   hipStream_t w2 = w[kWorkers - 1];
+  {
+    Span sp(d, s, FAM_TABLES);
+    launch_ang(d, s);
+  }
   if (P.ntot2 > 0) {
     launch_type2(d, w2);
     (void)hipEventRecord(d->t2done, w2);
+    (void)hipStreamWaitEvent(w2, d->angdone, 0);
     {
       Span sp(d, w2, FAM_ASM);
       launch_asm2(d, w2, 1, false);
@@ -2459,14 +2492,20 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     }
   }
   // ---- type 2 ----
+  {
+    Span sp(d, s, FAM_TABLES);
+    launch_ang(d, s);
+  }
   if (P.ntot2 > 0) {
     launch_type2(d, w[2]);
     (void)hipEventRecord(d->t2done, w[2]);
+    (void)hipStreamWaitEvent(w[2], d->angdone, 0);
     {
       Span sp(d, w[2], FAM_ASM);
       launch_asm2(d, w[2], 1, true);
     }
     (void)hipStreamWaitEvent(w[3], d->t2done, 0);
+    (void)hipStreamWaitEvent(w[3], d->angdone, 0);
     T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.ngroups, P.gflag, P.flist};
     const ClassMap cm2 = class_map(kMaxLMB + 1, P.lmb_count, [&](int c) {
       return (int)std::min<long>((long)P.lmb_count[c] * P.maxPairs, 8192);
