@@ -1568,6 +1568,7 @@ struct AsmArgs {
   // dynamic LDS layout (doubles): J is shared; the type-1 region (rsph, H)
   // and the type-2 region (blk, T) overlap
   int lds_blk, lds_H, lds_J, lds_rsph, lds_T, lds_rs_stride, pair_chunk, lds_Q;
+  int lds_S1, lds_S2, shift_lm;  // shift factor tables (aliasing rsph/Q resp. T) for shells up to shift_lm
 };
 
 // binomial shift of a block over C-centred monomials to the IJK(la) x IJK(lb)
@@ -1577,64 +1578,108 @@ struct V3 {
   double x, y, z;
 };
 
-template <typename Get>
-__device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V3 rA, const V3 rB, double norm,
-                            double *Jbuf, double *out) {
+// Nonzero binomial factors of the shift (util.c:246-334), listed once per
+// triplet: for component c of shell l (ordering position), the factors
+// C(a,alpha) (C-A)^(a-alpha) over alpha in the loop order of util.c:277-320,
+// each formed exactly as the per-element loops would (powers by repeated
+// multiplication, binomials left to right, factors |f| <= accuracy dropped,
+// the norm applied to the B side), with the position of alpha.
+__host__ __device__ constexpr int shift_mt(int l) {
+  int best = 1;
+  for (int ax = 0; ax <= l; ax++)
+    for (int ay = 0; ax + ay <= l; ay++) {
+      const int v = (ax + 1) * (ay + 1) * (l - ax - ay + 1);
+      best = v > best ? v : best;
+    }
+  return best;
+}
+
+// doubles of LDS for the factor tables of both sides for shells up to LM
+__host__ __device__ constexpr int shift_tab_doubles(int LM) {
+  const int n = ECP_DIM_IJK(LM) * shift_mt(LM);
+  return 2 * (n + (n + 1) / 2 + (ECP_DIM_IJK(LM) + 1) / 2);
+}
+
+struct ShiftTab {
+  double *f;
+  int *r, *n;
+  int mt;
+};
+
+__device__ __forceinline__ ShiftTab shift_tab_at(double *base, int LM, int side) {
+  const int n = ECP_DIM_IJK(LM) * shift_mt(LM);
+  double *b = base + side * (n + (n + 1) / 2 + (ECP_DIM_IJK(LM) + 1) / 2);
+  ShiftTab S;
+  S.f = b;
+  S.r = (int *)(b + n);
+  S.n = (int *)(b + n + (n + 1) / 2);
+  S.mt = shift_mt(LM);
+  return S;
+}
+
+__device__ void shift_factors(const DevTables &T, int l, const V3 v, bool with_norm, double norm, ShiftTab S) {
   const int D = T.ijkDim;
-  const int ncb = ECP_DIM_C(lb), nia = ECP_DIM_IJK(la), nib = ECP_DIM_IJK(lb);
   const double tol = T.accuracy;
-  for (int idx = threadIdx.x; idx < nia * ncb; idx += blockDim.x) {
-    const int c1 = idx / ncb, q = idx % ncb;
-    const int *e = T.ijk + 3 * ECP_IDX_C(la, c1);
+  for (int c = threadIdx.x; c < ECP_DIM_IJK(l); c += blockDim.x) {
+    const int *e = T.ijk + 3 * ECP_IDX_C(l, c);
     const int ax = e[0], ay = e[1], az = e[2];
-    double acc = 0.0;
+    int n = 0;
     for (int alx = 0; alx <= ax; alx++) {
-      // (C - A)^(a - alpha) as unitarySpherePolynomials (util.c:214-243)
       double x = 1.0;
-      for (int i = 0; i < ax - alx; i++) x = -x * rA.x;
+      for (int i = 0; i < ax - alx; i++) x = -x * v.x;
       const double bx = T.binom[ax * 16 + alx];
       for (int aly = 0; aly <= ay; aly++) {
         double y = 1.0;
-        for (int i = 0; i < ay - aly; i++) y = -y * rA.y;
+        for (int i = 0; i < ay - aly; i++) y = -y * v.y;
         const double by = bx * T.binom[ay * 16 + aly];
         for (int alz = 0; alz <= az; alz++) {
           double z = 1.0;
-          for (int i = 0; i < az - alz; i++) z = -z * rA.z;
+          for (int i = 0; i < az - alz; i++) z = -z * v.z;
           const double bz = by * T.binom[az * 16 + alz];
-          const double factor = bz * ((x * y) * z);
+          double factor = bz * ((x * y) * z);
           if (fabs(factor) <= tol) continue;
-          acc += factor * get(T.ijkIndex[(alx * D + aly) * D + alz], q);
+          if (with_norm) factor *= norm;
+          S.f[c * S.mt + n] = factor;
+          S.r[c * S.mt + n] = T.ijkIndex[(alx * D + aly) * D + alz];
+          n++;
         }
       }
     }
+    S.n[c] = n;
+  }
+}
+
+// binomial shift of a block over C-centred monomials to the IJK(la) x IJK(lb)
+// output (util.c:246-334).  The block is read through get(r, q): row r is an
+// ordering position of degree <= la, column q a position of degree <= lb.
+// tab: LDS for the factor tables (shift_tab_doubles(LM) doubles, LM >= la,
+// lb), free to overwrite; Jbuf: LDS for the half-shifted block.
+template <typename Get>
+__device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V3 rA, const V3 rB, double norm,
+                            double *Jbuf, double *tab, int LM, double *out) {
+  const int ncb = ECP_DIM_C(lb), nia = ECP_DIM_IJK(la), nib = ECP_DIM_IJK(lb);
+  const ShiftTab SA = shift_tab_at(tab, LM, 0), SB = shift_tab_at(tab, LM, 1);
+  __syncthreads();
+  shift_factors(T, la, rA, false, 1.0, SA);
+  shift_factors(T, lb, rB, true, norm, SB);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < nia * ncb; idx += blockDim.x) {
+    const int c1 = idx / ncb, q = idx % ncb;
+    const double *f = SA.f + c1 * SA.mt;
+    const int *r = SA.r + c1 * SA.mt;
+    const int n = SA.n[c1];
+    double acc = 0.0;
+    for (int t = 0; t < n; t++) acc += f[t] * get(r[t], q);
     Jbuf[idx] = acc;
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < nia * nib; idx += blockDim.x) {
     const int c1 = idx / nib, c2 = idx % nib;
-    const int *e = T.ijk + 3 * ECP_IDX_C(lb, c2);
-    const int bx_ = e[0], by_ = e[1], bz_ = e[2];
+    const double *f = SB.f + c2 * SB.mt;
+    const int *r = SB.r + c2 * SB.mt;
+    const int n = SB.n[c2];
     double acc = 0.0;
-    for (int btx = 0; btx <= bx_; btx++) {
-      double x = 1.0;
-      for (int i = 0; i < bx_ - btx; i++) x = -x * rB.x;
-      const double fx = T.binom[bx_ * 16 + btx];
-      for (int bty = 0; bty <= by_; bty++) {
-        double y = 1.0;
-        for (int i = 0; i < by_ - bty; i++) y = -y * rB.y;
-        const double fy = fx * T.binom[by_ * 16 + bty];
-        for (int btz = 0; btz <= bz_; btz++) {
-          double z = 1.0;
-          for (int i = 0; i < bz_ - btz; i++) z = -z * rB.z;
-          const double fz = fy * T.binom[bz_ * 16 + btz];
-          double factor = fz * ((x * y) * z);
-          if (fabs(factor) <= tol) continue;
-          factor *= norm;
-          const int r = T.ijkIndex[(btx * D + bty) * D + btz];
-          acc += factor * Jbuf[c1 * ncb + r];
-        }
-      }
-    }
+    for (int t = 0; t < n; t++) acc += f[t] * Jbuf[c1 * ncb + r[t]];
     out[idx] = acc;
   }
   __syncthreads();
@@ -1723,7 +1768,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
     const int *ea = T.ijk + 3 * r, *eb = T.ijk + 3 * q;
     return H[T.ijkIndex[((ea[0] + eb[0]) * D + ea[1] + eb[1]) * D + ea[2] + eb[2]]];
   };
-  shift_block(T, get1, tr.oa, tr.ob, vA, vB, norm1, Jbuf, A.blocks + tr.blk1);
+  shift_block(T, get1, tr.oa, tr.ob, vA, vB, norm1, Jbuf, lds + A.lds_S1, A.shift_lm, A.blocks + tr.blk1);
 }
 
 // Type-2 block of one triplet per wave: for each projector l, the T_l
@@ -1832,7 +1877,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm2(DevTables T, DevJob J, Asm
   }
   const double norm2 = (4.0 * M_PI) * (4.0 * M_PI);
   auto get2 = [&](int r, int q) { return blk[r * ncb + q]; };
-  shift_block(T, get2, tr.oa, tr.ob, vA, vB, norm2, Jbuf, A.blocks + tr.blk2);
+  shift_block(T, get2, tr.oa, tr.ob, vA, vB, norm2, Jbuf, lds + A.lds_S2, A.shift_lm, A.blocks + tr.blk2);
 }
 
 // ---------------------------------------------------------------------------
@@ -2695,17 +2740,24 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     aa.pair_chunk = std::min((LM <= 4) ? 16 : 8, std::max(1, P.maxPairs));
     aa.lds_rs_stride = (2 * LM + 1) * (2 * LM + 1);
     int o = 0;
+    aa.shift_lm = LM;
+    const int ns = shift_tab_doubles(LM);
     aa.lds_J = o; o += ECP_DIM_IJK(LM) * ECP_DIM_C(LM);
     const int base = o;
-    aa.lds_rsph = o; o += aa.pair_chunk * aa.lds_rs_stride;
+    // type 1: H stays for the shift; the shift tables reuse rsph and Q
     aa.lds_H = o; o += ECP_DIM_C(2 * LM);
+    aa.lds_S1 = o;
+    aa.lds_rsph = o; o += aa.pair_chunk * aa.lds_rs_stride;
     aa.lds_Q = o; o += aa.pair_chunk * nq_of(2 * LM);
-    const int end1 = o;
+    const int end1 = std::max(o, aa.lds_S1 + ns);
+    // type 2: blk stays for the shift; the shift tables reuse T
     o = base;
     aa.lds_blk = o; o += ECP_DIM_C(LM) * ECP_DIM_C(LM);
+    aa.lds_S2 = o;
     aa.lds_T = o; o += (LM + LC) * (LM + LC) * (2 * LM + 1);
+    const int end2 = std::max(o, aa.lds_S2 + ns);
     P.asm1_lds = (size_t)end1 * sizeof(double);
-    P.asm2_lds = (size_t)o * sizeof(double);
+    P.asm2_lds = (size_t)end2 * sizeof(double);
     if (std::max(P.asm1_lds, P.asm2_lds) > 64 * 1024) {
       snprintf(err, errlen, "assembly needs %zu bytes of LDS (l too high for this build)",
                std::max(P.asm1_lds, P.asm2_lds));
