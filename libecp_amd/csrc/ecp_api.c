@@ -467,6 +467,16 @@ int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n) {
   return ecp_dev_last_timing(b->dev, ms, n);
 }
 
+int libecp_batch_quadrature_ms(libecp_batch *b, double *ms) {
+  char err[512] = "";
+  if (!b || !ms) return -1;
+  if (ecp_dev_span_ms(b->dev, ms, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  return 0;
+}
+
 int libecp_batch_profile(libecp_batch *b, int *launches, double *ms, double *flops, int n) {
   char err[512] = "";
   int l[16];

@@ -1993,6 +1993,11 @@ struct ecp_dev {
   hipStream_t stream = nullptr;
   hipStream_t workers[kWorkers] = {};
   hipEvent_t fork = nullptr, t2done = nullptr, t1done = nullptr, angdone = nullptr, join[kWorkers] = {};
+  // the type-1 small-grid quadrature (head and tail launches) of the last
+  // evaluation, recorded on its stream also inside graph replays
+  hipEvent_t span[2] = {};
+  bool span_valid = false;
+  bool capturing = false;  // enqueue runs under stream capture
   DevTables T{};
   void *static_mem = nullptr;
   char *arena = nullptr;
@@ -2122,6 +2127,8 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
   if ((e = hipEventCreateWithFlags(&d->t2done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->t1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->angdone, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
+  for (int i = 0; i < 2; i++)
+    if ((e = hipEventCreate(&d->span[i])) != hipSuccess) return fail("hipEventCreate", e);
   for (int i = 0; i < 8; i++)
     if ((e = hipEventCreate(&d->ev[i])) != hipSuccess) return fail("hipEventCreate", e);
   build_templates(d);
@@ -2227,6 +2234,8 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   if (d->t2done) (void)hipEventDestroy(d->t2done);
   if (d->t1done) (void)hipEventDestroy(d->t1done);
   if (d->angdone) (void)hipEventDestroy(d->angdone);
+  for (int i = 0; i < 2; i++)
+    if (d->span[i]) (void)hipEventDestroy(d->span[i]);
   for (auto e : d->pev) (void)hipEventDestroy(e);
   if (d->con_mem) (void)hipFree(d->con_mem);
   for (int i = 0; i < 2; i++)
@@ -2380,6 +2389,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   const DevTables &T = d->T;
   const DevJob &J = P.J;
   hipStream_t *w = d->workers;
+  d->span_valid = false;  // type-1 classes run on several streams
   if (evs) (void)hipEventRecord(evs[0], s);
   {
     Span sp(d, s, FAM_TABLES);
@@ -2471,6 +2481,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   const DevTables &T = d->T;
   const DevJob &J = P.J;
   hipStream_t *w = d->workers;
+  d->span_valid = false;
   if (evs) (void)hipEventRecord(evs[0], s);
   {
     Span sp(d, s, FAM_TABLES);
@@ -2507,6 +2518,8 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   const bool big1 = P.max_lab > 6;
   const ClassMap cmh = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return (P.cls_count[c] + 63) / 64; });
   if (cmh.n > 0) {
+    // external event nodes when captured, so that replays record them too
+    (void)hipEventRecordWithFlags(d->span[0], w[0], d->capturing ? hipEventRecordExternal : 0);
     {
       Span sp(d, w[0], FAM_TYPE1);
       if (big1) k_type1_fused<10, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
@@ -2519,6 +2532,8 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       if (big1) k_t1tail_fused<10><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
       else k_t1tail_fused<6><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
     }
+    (void)hipEventRecordWithFlags(d->span[1], w[0], d->capturing ? hipEventRecordExternal : 0);
+    d->span_valid = true;
     (void)hipEventRecord(d->t1done, w[0]);
     {
       Span sp(d, w[0], FAM_ASM);
@@ -2924,6 +2939,9 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
   int remaining = repeats;
   double total = 0.0;
   if (!d->eager_done) {
+    // a stale error of an earlier, unrelated call must not be taken for one
+    // of this enqueue's launches
+    (void)hipGetLastError();
     enqueue(d, s, d->ev);
     ECP_CHECK(hipGetLastError());
     ECP_CHECK(hipStreamSynchronize(s));
@@ -2939,7 +2957,9 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
   }
   if (remaining > 0 && !d->gexec) {
     ECP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    d->capturing = true;
     enqueue(d, s, nullptr);
+    d->capturing = false;
     ECP_CHECK(hipStreamEndCapture(s, &d->graph));
     ECP_CHECK(hipGraphInstantiate(&d->gexec, d->graph, nullptr, nullptr, 0));
   }
@@ -3264,6 +3284,17 @@ extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P,
     ECP_CHECK(hipEventElapsedTime(&f, d->con_ev[0], d->con_ev[1]));
     *ms = f;
   }
+  return 0;
+}
+
+extern "C" int ecp_dev_span_ms(ecp_dev *d, double *ms, char *err, size_t errlen) {
+  if (!d->prepared || !d->span_valid) {
+    snprintf(err, errlen, "no single-stream type-1 quadrature span in the last evaluation");
+    return -1;
+  }
+  float f = 0.f;
+  ECP_CHECK(hipEventElapsedTime(&f, d->span[0], d->span[1]));
+  *ms = f;
   return 0;
 }
 

@@ -267,6 +267,14 @@ class Batch:
         rc = lib().libecp_batch_matrix(C.c_void_p(self.b), C.c_int(i), C.c_int(m.nao), _P(I, C.c_double))
         return rc, I
 
+    def quadrature_ms(self):
+        """libecp_batch_quadrature_ms: the type-1 small-grid quadrature's
+        device time in the last evaluation (None with per-class launches)."""
+        ms = C.c_double()
+        if lib().libecp_batch_quadrature_ms(C.c_void_p(self.b), C.byref(ms)) != 0:
+            return None
+        return ms.value
+
     def triplets(self):
         ev = C.c_longlong()
         nom = lib().libecp_batch_triplets(C.c_void_p(self.b), C.byref(ev))

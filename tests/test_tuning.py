@@ -76,3 +76,24 @@ def test_unknown_parameter_is_rejected():
     L = native.lib()
     assert L.libecp_batch_set_param(None, b"t1_head_coop", 4) == -1
     assert L.libecp_batch_get_param(None, b"nope", C.byref(C.c_int())) == -1
+
+
+@pytest.mark.gpu
+def test_quadrature_span_inside_graph_replays():
+    """The type-1 quadrature's event pair is recorded by graph replays too:
+    after replays it reads a positive span shorter than the evaluation, and
+    per-class launches report none."""
+    b = native.Batch([medium_cluster(0)])
+    try:
+        b.set_param("fuse", 1)
+        b.execute(1)                     # eager
+        eager = b.quadrature_ms()
+        per_step = b.execute(5)          # graph replays
+        replay = b.quadrature_ms()
+        assert eager is not None and replay is not None
+        assert 0.0 < replay < per_step
+        b.set_param("fuse", 0)
+        b.execute(2)
+        assert b.quadrature_ms() is None
+    finally:
+        b.close()
