@@ -328,12 +328,10 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     value = nominal / (wall / args.steps)
-    # the type-1 quadrature's duration inside graph replays: the last replay
-    # of the timed region, then further single replays read one by one
-    quad = [b.quadrature_ms()]
-    for _ in range(10):
-        b.execute(1)
-        quad.append(b.quadrature_ms())
+    # the type-1 quadrature's duration inside the timed region's graph
+    # replays (mean over its last replays, one per captured graph copy)
+    quad_ms = b.quadrature_ms()
+    quad_n = getattr(b, "quadrature_samples", 0)
     b.fetch()
     rc, I_gpu = b.matrix(0)
     b.close()
@@ -380,10 +378,9 @@ def main():
     fams = {"type1": (t1_launch, t1_ms + tail_ms, t1_flops), "type2": prof["type2"]}
     fam = max((k for k in fams if fams[k][0] > 0), key=lambda k: fams[k][1])
     launches, kms, kflops = fams[fam]
-    quad_ok = fam == "type1" and all(q is not None for q in quad)
+    quad_ok = fam == "type1" and quad_ms is not None
     if quad_ok:
-        # graph replays, the configuration of the timed region
-        kms = statistics.mean(quad)
+        kms = quad_ms
     achieved = kflops / (kms * 1e-3) / 1e12
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": achieved / FP64_PEAK_TFLOPS,
@@ -396,9 +393,8 @@ def main():
                                "the kernels evaluate points lazily up to convergence"}
     if fam == "type1":
         out["roofline"]["duration_source"] = (
-            "HIP events on the quadrature's stream inside graph replays (last replay of the timed region "
-            "and 10 more), mean" if quad_ok else "one eager evaluation, HIP events per launch")
-        out["roofline"]["replay_ms_samples"] = quad if quad_ok else None
+            f"HIP events on the quadrature's stream inside the timed region's graph replays (mean of its last "
+            f"{quad_n})" if quad_ok else "one eager evaluation, HIP events per launch")
         out["roofline"]["eager_ms_k_type1"] = t1_ms / max(t1_launch, 1)
         out["roofline"]["eager_ms_k_t1tail"] = tail_ms / max(tail_launch, 1)
     step_flops = prof["type1"][2] + prof["type2"][2]

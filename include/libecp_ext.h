@@ -98,10 +98,13 @@ int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n);
 int libecp_batch_profile(libecp_batch *b, int *launches, double *ms, double *flops, int n);
 const char *libecp_kernel_family(int i);
 /* Device time (ms) of the type-1 small-grid quadrature -- the head and tail
-   launches, from an event pair on their stream -- in the last evaluation,
-   graph replays included.  0, or -1 when the last evaluation used per-class
+   launches, from an event pair on their stream -- in the last
+   libecp_batch_execute call: its eager evaluation, or the mean over its
+   last graph replays (up to 8; the graph is captured in copies with their
+   own event pair and replays cycle over them).  *samples (may be NULL):
+   evaluations averaged.  0, or -1 when the evaluation used per-class
    launches (the quadrature then spans several streams). */
-int libecp_batch_quadrature_ms(libecp_batch *b, double *ms);
+int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples);
 /* Launch parameters of the batch's device pipeline, by name:
    "t1_head_coop" / "t1_head_serial" (PS93 levels the type-1 head kernel runs
    before the wave-cooperative / lane-serial tail), "t1_coop_max" (largest

@@ -467,10 +467,10 @@ int libecp_batch_stage_ms(libecp_batch *b, double *ms, int n) {
   return ecp_dev_last_timing(b->dev, ms, n);
 }
 
-int libecp_batch_quadrature_ms(libecp_batch *b, double *ms) {
+int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples) {
   char err[512] = "";
   if (!b || !ms) return -1;
-  if (ecp_dev_span_ms(b->dev, ms, err, sizeof(err))) {
+  if (ecp_dev_span_ms(b->dev, ms, samples, err, sizeof(err))) {
     set_last_error(err);
     return -1;
   }

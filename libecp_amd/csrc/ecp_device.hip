@@ -1988,16 +1988,23 @@ struct Prepared {
 }  // namespace
 
 constexpr int kWorkers = 4;
+constexpr int kSpanGraphs = 8;  // graph copies with their own quadrature event pair
 
 struct ecp_dev {
   hipStream_t stream = nullptr;
   hipStream_t workers[kWorkers] = {};
   hipEvent_t fork = nullptr, t2done = nullptr, t1done = nullptr, angdone = nullptr, join[kWorkers] = {};
-  // the type-1 small-grid quadrature (head and tail launches) of the last
-  // evaluation, recorded on its stream also inside graph replays
-  hipEvent_t span[2] = {};
+  // the type-1 small-grid quadrature (head and tail launches), recorded on
+  // its stream: event pair k belongs to captured graph k (replays cycle over
+  // the graphs, so the last kSpanGraphs replays can be read); the eager
+  // evaluation uses pair 0
+  hipEvent_t span[kSpanGraphs][2] = {};
+  int span_k = 0;
   bool span_valid = false;
   bool capturing = false;  // enqueue runs under stream capture
+  bool last_eager = false;  // the last execute ran no replay
+  long long replays = 0;    // replays since the graphs were captured
+  int last_replays = 0;     // replays of the last execute
   DevTables T{};
   void *static_mem = nullptr;
   char *arena = nullptr;
@@ -2016,8 +2023,9 @@ struct ecp_dev {
   JobLayout L;
   ecp_tuning tune{8, 15, 100000, 8192, 2, 2, 15, 2048};
   bool planned = false, prepared = false, eager_done = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph[kSpanGraphs] = {};
+  hipGraphExec_t gexec[kSpanGraphs] = {};
+  int ngraph = 0;
   // per-launch profiling (eager runs only)
   bool profiling = false;
   std::vector<hipEvent_t> pev;
@@ -2094,10 +2102,14 @@ void build_templates(ecp_dev *d) {
 }
 
 void reset_graph(ecp_dev *d) {
-  if (d->gexec) (void)hipGraphExecDestroy(d->gexec);
-  if (d->graph) (void)hipGraphDestroy(d->graph);
-  d->gexec = nullptr;
-  d->graph = nullptr;
+  for (int k = 0; k < d->ngraph; k++) {
+    if (d->gexec[k]) (void)hipGraphExecDestroy(d->gexec[k]);
+    if (d->graph[k]) (void)hipGraphDestroy(d->graph[k]);
+    d->gexec[k] = nullptr;
+    d->graph[k] = nullptr;
+  }
+  d->ngraph = 0;
+  d->replays = 0;
 }
 
 }  // namespace
@@ -2127,8 +2139,9 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
   if ((e = hipEventCreateWithFlags(&d->t2done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->t1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->angdone, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
-  for (int i = 0; i < 2; i++)
-    if ((e = hipEventCreate(&d->span[i])) != hipSuccess) return fail("hipEventCreate", e);
+  for (int k = 0; k < kSpanGraphs; k++)
+    for (int i = 0; i < 2; i++)
+      if ((e = hipEventCreate(&d->span[k][i])) != hipSuccess) return fail("hipEventCreate", e);
   for (int i = 0; i < 8; i++)
     if ((e = hipEventCreate(&d->ev[i])) != hipSuccess) return fail("hipEventCreate", e);
   build_templates(d);
@@ -2234,8 +2247,9 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   if (d->t2done) (void)hipEventDestroy(d->t2done);
   if (d->t1done) (void)hipEventDestroy(d->t1done);
   if (d->angdone) (void)hipEventDestroy(d->angdone);
-  for (int i = 0; i < 2; i++)
-    if (d->span[i]) (void)hipEventDestroy(d->span[i]);
+  for (int k = 0; k < kSpanGraphs; k++)
+    for (int i = 0; i < 2; i++)
+      if (d->span[k][i]) (void)hipEventDestroy(d->span[k][i]);
   for (auto e : d->pev) (void)hipEventDestroy(e);
   if (d->con_mem) (void)hipFree(d->con_mem);
   for (int i = 0; i < 2; i++)
@@ -2519,7 +2533,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   const ClassMap cmh = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return (P.cls_count[c] + 63) / 64; });
   if (cmh.n > 0) {
     // external event nodes when captured, so that replays record them too
-    (void)hipEventRecordWithFlags(d->span[0], w[0], d->capturing ? hipEventRecordExternal : 0);
+    (void)hipEventRecordWithFlags(d->span[d->span_k][0], w[0], d->capturing ? hipEventRecordExternal : 0);
     {
       Span sp(d, w[0], FAM_TYPE1);
       if (big1) k_type1_fused<10, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
@@ -2532,7 +2546,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       if (big1) k_t1tail_fused<10><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
       else k_t1tail_fused<6><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
     }
-    (void)hipEventRecordWithFlags(d->span[1], w[0], d->capturing ? hipEventRecordExternal : 0);
+    (void)hipEventRecordWithFlags(d->span[d->span_k][1], w[0], d->capturing ? hipEventRecordExternal : 0);
     d->span_valid = true;
     (void)hipEventRecord(d->t1done, w[0]);
     {
@@ -2938,10 +2952,13 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
   hipStream_t s = d->stream;
   int remaining = repeats;
   double total = 0.0;
+  d->last_eager = false;
   if (!d->eager_done) {
     // a stale error of an earlier, unrelated call must not be taken for one
     // of this enqueue's launches
     (void)hipGetLastError();
+    d->span_k = 0;
+    d->last_eager = true;
     enqueue(d, s, d->ev);
     ECP_CHECK(hipGetLastError());
     ECP_CHECK(hipStreamSynchronize(s));
@@ -2955,17 +2972,36 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
     d->eager_done = true;
     remaining--;
   }
-  if (remaining > 0 && !d->gexec) {
-    ECP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    d->capturing = true;
-    enqueue(d, s, nullptr);
-    d->capturing = false;
-    ECP_CHECK(hipStreamEndCapture(s, &d->graph));
-    ECP_CHECK(hipGraphInstantiate(&d->gexec, d->graph, nullptr, nullptr, 0));
+  if (remaining > 0 && d->ngraph == 0) {
+    // fused launches: kSpanGraphs copies, each recording its own quadrature
+    // event pair; per-class launches: one graph
+    for (int k = 0; k < kSpanGraphs; k++) {
+      d->span_k = k;
+      // a first host-side record, so that the pair reads as recorded before
+      // the graph's event nodes overwrite it
+      ECP_CHECK(hipEventRecord(d->span[k][0], s));
+      ECP_CHECK(hipEventRecord(d->span[k][1], s));
+      ECP_CHECK(hipStreamSynchronize(s));
+      ECP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      d->capturing = true;
+      enqueue(d, s, nullptr);
+      d->capturing = false;
+      ECP_CHECK(hipStreamEndCapture(s, &d->graph[k]));
+      ECP_CHECK(hipGraphInstantiate(&d->gexec[k], d->graph[k], nullptr, nullptr, 0));
+      d->ngraph = k + 1;
+      if (!d->span_valid) break;
+    }
+    d->span_k = 0;
+    d->replays = 0;
   }
   if (remaining > 0) {
+    d->last_eager = false;
+    d->last_replays = remaining;
     ECP_CHECK(hipEventRecord(d->ev[5], s));
-    for (int r = 0; r < remaining; r++) ECP_CHECK(hipGraphLaunch(d->gexec, s));
+    for (int r = 0; r < remaining; r++) {
+      ECP_CHECK(hipGraphLaunch(d->gexec[d->replays % d->ngraph], s));
+      d->replays++;
+    }
     ECP_CHECK(hipEventRecord(d->ev[6], s));
     ECP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;
@@ -2993,6 +3029,9 @@ extern "C" int ecp_dev_profile(ecp_dev *d, int *launches, double *ms, double *fl
   if (d->P.nt == 0) return FAM_N;
   d->profiling = true;
   d->npev = 0;
+  d->span_k = 0;
+  d->last_eager = true;
+  (void)hipGetLastError();
   enqueue(d, d->stream, nullptr);
   d->profiling = false;
   ECP_CHECK(hipGetLastError());
@@ -3287,14 +3326,22 @@ extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P,
   return 0;
 }
 
-extern "C" int ecp_dev_span_ms(ecp_dev *d, double *ms, char *err, size_t errlen) {
-  if (!d->prepared || !d->span_valid) {
+extern "C" int ecp_dev_span_ms(ecp_dev *d, double *ms, int *samples, char *err, size_t errlen) {
+  if (!d->prepared || !d->span_valid || (!d->last_eager && d->replays == 0)) {
     snprintf(err, errlen, "no single-stream type-1 quadrature span in the last evaluation");
     return -1;
   }
-  float f = 0.f;
-  ECP_CHECK(hipEventElapsedTime(&f, d->span[0], d->span[1]));
-  *ms = f;
+  // the eager evaluation, or the last replays (one per graph copy)
+  const int n = d->last_eager ? 1 : std::min(d->ngraph, d->last_replays);
+  double sum = 0.0;
+  for (int i = 0; i < n; i++) {
+    const int k = d->last_eager ? 0 : (int)((d->replays - 1 - i) % d->ngraph);
+    float f = 0.f;
+    ECP_CHECK(hipEventElapsedTime(&f, d->span[k][0], d->span[k][1]));
+    sum += f;
+  }
+  *ms = sum / n;
+  if (samples) *samples = n;
   return 0;
 }
 

@@ -128,9 +128,11 @@ const char *ecp_dev_family_name(int f);
    and large-grid fallbacks, concurrent), assembly[, mean graph replay]. */
 int ecp_dev_last_timing(ecp_dev *d, double *ms, int n);
 /* Device time (ms) from the start of the type-1 head launch to the end of
-   the type-1 tail launch in the last evaluation (events on their stream,
-   recorded inside graph replays too); fused launches only, else -1. */
-int ecp_dev_span_ms(ecp_dev *d, double *ms, char *err, size_t errlen);
+   the type-1 tail launch (events on their stream, recorded inside graph
+   replays too): the last execute's eager evaluation, or the mean over its
+   last replays (up to 8, one per captured graph copy; *samples = count);
+   fused launches only, else -1. */
+int ecp_dev_span_ms(ecp_dev *d, double *ms, int *samples, char *err, size_t errlen);
 /* launch parameters: defaults at creation; setting them drops a captured
    graph (the next execute runs eagerly and captures again) */
 void ecp_dev_get_tuning(const ecp_dev *d, ecp_tuning *t);

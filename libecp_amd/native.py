@@ -270,9 +270,10 @@ class Batch:
     def quadrature_ms(self):
         """libecp_batch_quadrature_ms: the type-1 small-grid quadrature's
         device time in the last evaluation (None with per-class launches)."""
-        ms = C.c_double()
-        if lib().libecp_batch_quadrature_ms(C.c_void_p(self.b), C.byref(ms)) != 0:
+        ms, n = C.c_double(), C.c_int()
+        if lib().libecp_batch_quadrature_ms(C.c_void_p(self.b), C.byref(ms), C.byref(n)) != 0:
             return None
+        self.quadrature_samples = n.value
         return ms.value
 
     def triplets(self):

@@ -88,10 +88,13 @@ def test_quadrature_span_inside_graph_replays():
         b.set_param("fuse", 1)
         b.execute(1)                     # eager
         eager = b.quadrature_ms()
+        assert eager is not None and b.quadrature_samples == 1
         per_step = b.execute(5)          # graph replays
         replay = b.quadrature_ms()
-        assert eager is not None and replay is not None
+        assert replay is not None and b.quadrature_samples == 5
         assert 0.0 < replay < per_step
+        b.execute(12)                    # replays cycle over the 8 graph copies
+        assert b.quadrature_ms() is not None and b.quadrature_samples == 8
         b.set_param("fuse", 0)
         b.execute(2)
         assert b.quadrature_ms() is None
