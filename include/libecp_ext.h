@@ -115,7 +115,10 @@ int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples);
    for small jobs), "asm2_cache" (type-2 assembly with the angular factors
    in registers: 0 no, 1 yes, 2, the default: with fused launches),
    "t2_head" (PS93 levels of the type-2 small-grid kernel before the entries
-   still open are rerun compacted; 15: no split), "t2_tail_grid".  Results do
+   still open are rerun compacted; 15: no split), "t2_tail_grid",
+   "fb2_split" (waves sharing one (triplet, primitive pair) of the type-2
+   large-grid fallback; 0, the default: 3 for jobs of up to a million
+   (triplet, pair) items, 1 for larger ones).  Results do
    not depend on them.  0, or -1 for an unknown name. */
 int libecp_batch_set_param(libecp_batch *b, const char *name, int value);
 int libecp_batch_get_param(libecp_batch *b, const char *name, int *value);

@@ -509,6 +509,7 @@ static int *tuning_field(ecp_tuning *t, const char *name) {
   if (!strcmp(name, "asm2_cache")) return &t->asm2_cache;
   if (!strcmp(name, "t2_head")) return &t->t2_head;
   if (!strcmp(name, "t2_tail_grid")) return &t->t2_tail_grid;
+  if (!strcmp(name, "fb2_split")) return &t->fb2_split;
   return NULL;
 }
 
@@ -577,6 +578,7 @@ int libecp_batch_autotune(libecp_batch *b, int repeats, char *report, int len) {
       {"t1_head_coop", 4, {6, 8, 10, 12}},
       {"t1_head_serial", 4, {6, 8, 10, 15}},
       {"t1_tail_grid", 4, {2048, 8192, 32768, 131072}},
+      {"fb2_split", 4, {0, 1, 2, 3}},
   };
   for (size_t k = 0; k < sizeof(space) / sizeof(space[0]); k++) {
     for (int i = 0; i < space[k].n; i++) {

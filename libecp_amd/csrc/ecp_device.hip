@@ -1096,11 +1096,13 @@ struct T2Args {
   const int *tmpl;
   double *Tout;
   unsigned char *tfail;
-  int *gflag, *fbq, *fbcount;
+  int *gflag, *fbq, *fbcount;  // fbq: per class, the triplets with failed entries (qstride apart)
   int qstride;
   unsigned char *fbflag;
   int *flist;
   int *tq, *tcount;  // entries left open by the head
+  int *tclaim;                  // per triplet: queued for the fallback
+  const unsigned char *tcls2;   // per triplet: its fallback class max(la, lb) + L - 1
 };
 
 __device__ __forceinline__ void t2_store(const T2Args &A, int gid, int g, const T2Group &G, int st, double res) {
@@ -1111,9 +1113,12 @@ __device__ __forceinline__ void t2_store(const T2Args &A, int gid, int g, const 
     // flist[eoff ...]; the first one queues the group for the fallback
     const int k = atomicAdd(&A.gflag[g], 1);
     A.flist[G.eoff + k] = gid - G.eoff;
-    if (k == 0) {
-      const int slot = atomicAdd(&A.fbcount[G.lmb], 1);
-      A.fbq[(size_t)G.lmb * A.qstride + slot] = g;
+    // the triplet's first failed entry queues the triplet: the fallback
+    // takes all its failed groups together
+    if (k == 0 && atomicExch(&A.tclaim[G.trip], 1) == 0) {
+      const int c = A.tcls2[G.trip];
+      const int slot = atomicAdd(&A.fbcount[c], 1);
+      A.fbq[(size_t)c * A.qstride + slot] = G.trip;
       A.fbflag[G.trip] = 1;
     }
   }
@@ -1222,13 +1227,16 @@ __global__ __launch_bounds__(64) void k_type2_tail(DevTables T, DevJob J, T2Args
   }
 }
 
-// type-2 fallback (calcT_FM06, type2.c:417-528): one wave per (failed
-// (triplet, l) group, primitive pair).  The wave evaluates each level's points
-// (U_l, K_a, K_b, exponential) into LDS; lanes own the group's failed T
-// entries (up to 4 each, 256 per pass) and sum the points in the
-// integrator's pair order.  Results per (entry, pair) go to fbT and are summed
-// over pairs, in pair order, by the assembly.
+// type-2 fallback (calcT_FM06, type2.c:417-528): one wave per (triplet with
+// failed T entries, primitive pair).  The FM06-mapped points depend on the
+// pair only, so the wave evaluates each level's points once (K_a, K_b, the
+// exponential, and U_l for every channel l with failed entries) into LDS
+// for all failed entries of all the triplet's channels; lanes own the failed
+// entries (64 per pass) and sum the points in the integrator's pair order.
+// Results per (entry, pair) go to fbT and are summed over pairs, in pair
+// order, by the assembly.
 constexpr int kFb2PerLane = 1;
+constexpr int kFb2Cnt = 52;  // fbc[kFb2Cnt + c]: triplets queued for the type-2 fallback, class c
 
 template <int LMB>
 __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, const DevJob &J,
@@ -1236,15 +1244,17 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
                                          const unsigned char *__restrict__ tfail, const int *__restrict__ fbq,
                                          const int *__restrict__ fbcount, double *__restrict__ fbT,
                                          unsigned char *__restrict__ status, const int *__restrict__ gcount,
-                                         const int *__restrict__ flist, double *s_r, double *s_w, double *s_cu,
-                                         double *s_ex, double *s_ka, double *s_kb) {
+                                         const int *__restrict__ flist, const int *__restrict__ tgfirst,
+                                         int nsplit, double *s_r, double *s_w, double *s_cu, double *s_ex,
+                                         double *s_ka, double *s_kb) {
   const int lane = threadIdx.x;
-  const long total = (long)(*fbcount) * J.maxPairs;
+  // items (triplet, pair, split): split k takes the passes k, k + nsplit, ...
+  const long total = (long)(*fbcount) * J.maxPairs * nsplit;
   for (long it = bid; it < total; it += nb) {
-    const int g = fbq[it / J.maxPairs];
-    const int pair = (int)(it % J.maxPairs);
-    const T2Group G = groups[g];
-    const ecp_triplet tr = J.trip[G.trip];
+    const int t = fbq[it / ((long)J.maxPairs * nsplit)];
+    const int rem = (int)(it % ((long)J.maxPairs * nsplit));
+    const int pair = rem / nsplit, split = rem % nsplit;
+    const ecp_triplet tr = J.trip[t];
     const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
     if (pair >= Sa.K * Sb.K) continue;
     const int pa = pair / Sb.K, pb = pair % Sb.K;
@@ -1266,7 +1276,16 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     };
     const int brk = fm06_break(T, M, dAC, dBC, minExp, expo_of);
     const int end = brk - 1;
-    const int laC = tr.la + G.l, lbC = tr.lb + G.l;
+    // the triplet's groups (one per semi-local channel l) and their failed
+    // entries; the points are shared by all of them
+    const int g0 = tgfirst[t], L = Cn.L;
+    int gc[ECP_MAX_L], nfail = 0, lmax = 0;
+    for (int l = 0; l < L; l++) {
+      gc[l] = gcount[g0 + l];
+      nfail += gc[l];
+      if (gc[l]) lmax = l;
+    }
+    const int laC = tr.la + lmax, lbC = tr.lb + lmax;
     const ecp_gauss *gg = J.gauss + Cn.g0;
     auto eval = [&](int slot, int n) {
       double r = 0.0, w = 0.0, cu = 0.0, ex = 0.0;
@@ -1276,14 +1295,17 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
         const double expo = expo_of(r);
         if (expo >= minExp) {
           live = true;
-          cu = Cc * eval_ecp(gg, Cn.ng, G.l, r);
           ex = exp(expo);
           w = T.wl[n] * M.i1;
         }
       }
+      for (int l = 0; l < L; l++) {
+        if (!gc[l]) continue;
+        cu = live ? Cc * eval_ecp(gg, Cn.ng, l, r) : 0.0;
+        s_cu[l * 64 + slot] = cu;
+      }
       s_r[slot] = r;
       s_w[slot] = w;
-      s_cu[slot] = cu;
       s_ex[slot] = ex;
       // one Bessel vector live at a time (register pressure)
       {
@@ -1305,18 +1327,31 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
           if (l <= lbC) s_kb[l * kKS + slot] = K[l];
       }
     };
-    // the group's failed entries (listed by k_type2), 64 per pass
-    const int nfail = gcount[g];
-    for (int pass0 = 0; pass0 < nfail; pass0 += 64 * kFb2PerLane) {
+    // the failed entries of all groups (listed by k_type2), group after
+    // group, 64 per pass
+    if (split * 64 * kFb2PerLane >= nfail) continue;
+    for (int pass0 = split * 64 * kFb2PerLane; pass0 < nfail; pass0 += 64 * kFb2PerLane * nsplit) {
       const int nhere = min(64 * kFb2PerLane, nfail - pass0);
-      int ent[kFb2PerLane], c1[kFb2PerLane], c2[kFb2PerLane], c3[kFb2PerLane];
+      int ent[kFb2PerLane], c1[kFb2PerLane], c2[kFb2PerLane], c3[kFb2PerLane], el[kFb2PerLane];
+      int eo[kFb2PerLane];  // the entry's index in the job's T arrays
       double I[kFb2PerLane], p[kFb2PerLane], res[kFb2PerLane];
       unsigned int done = 0;
 #pragma unroll
       for (int j = 0; j < kFb2PerLane; j++) {
         const int k = lane + 64 * j;
-        ent[j] = (k < nhere) ? flist[G.eoff + pass0 + k] : -1;
-        const int code = ent[j] >= 0 ? tmpl[G.tmpl + ent[j]] : 0;
+        ent[j] = -1;
+        el[j] = 0;
+        eo[j] = 0;
+        int code = 0;
+        if (k < nhere) {
+          int kk = pass0 + k, l = 0;
+          while (kk >= gc[l]) kk -= gc[l++];
+          const T2Group G = groups[g0 + l];
+          ent[j] = flist[G.eoff + kk];
+          code = tmpl[G.tmpl + ent[j]];
+          el[j] = l;
+          eo[j] = G.eoff + ent[j];
+        }
         c1[j] = code & 0xff;
         c2[j] = (code >> 8) & 0xff;
         c3[j] = (code >> 16) & 0xff;
@@ -1327,7 +1362,8 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
       auto f = [&](int j, int slot) {
         double rn = 1.0;
         for (int k = 0; k < c3[j]; k++) rn = s_r[slot] * rn;
-        return s_w[slot] * ((((s_cu[slot] * rn) * s_ka[c1[j] * kKS + slot]) * s_kb[c2[j] * kKS + slot]) * s_ex[slot]);
+        return s_w[slot] *
+               ((((s_cu[el[j] * 64 + slot] * rn) * s_ka[c1[j] * kKS + slot]) * s_kb[c2[j] * kKS + slot]) * s_ex[slot]);
       };
       __syncthreads();
       auto test = [&](int l, const double (&q)[kFb2PerLane]) {
@@ -1408,8 +1444,8 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
 #pragma unroll
       for (int j = 0; j < kFb2PerLane; j++) {
         if (ent[j] < 0) continue;
-        if (done & (1u << j)) fbT[(size_t)(G.eoff + ent[j]) * J.maxPairs + pair] = res[j];
-        else status[2 * G.trip + 1] = 1;
+        if (done & (1u << j)) fbT[(size_t)eo[j] * J.maxPairs + pair] = res[j];
+        else status[2 * t + 1] = 1;
       }
     }
     __syncthreads();
@@ -1421,11 +1457,12 @@ __global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group
                                             const int *__restrict__ tmpl, const unsigned char *__restrict__ tfail,
                                             const int *__restrict__ fbq, const int *__restrict__ fbcount,
                                             double *__restrict__ fbT, unsigned char *__restrict__ status,
-                                            const int *__restrict__ gcount, const int *__restrict__ flist) {
-  __shared__ double s_r[64], s_w[64], s_cu[64], s_ex[64];
+                                            const int *__restrict__ gcount, const int *__restrict__ flist,
+                                            const int *__restrict__ tgfirst, int nsplit) {
+  __shared__ double s_r[64], s_w[64], s_cu[ECP_MAX_L * 64], s_ex[64];
   __shared__ double s_ka[(LMB + 1) * kKS], s_kb[(LMB + 1) * kKS];
-  fb2_body<LMB>(blockIdx.x, gridDim.x, T, J, groups, tmpl, tfail, fbq, fbcount, fbT, status, gcount, flist, s_r, s_w,
-                s_cu, s_ex, s_ka, s_kb);
+  fb2_body<LMB>(blockIdx.x, gridDim.x, T, J, groups, tmpl, tfail, fbq, fbcount, fbT, status, gcount, flist, tgfirst,
+                nsplit, s_r, s_w, s_cu, s_ex, s_ka, s_kb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1521,17 +1558,18 @@ struct T2Set {
   const T2Group *groups;
   const int *tmpl;
   const unsigned char *tf;
-  const int *fbq2;
+  const int *fbq2;  // per class, the triplets with failed entries (nt apart)
   const int *fbc;
   double *fbT;
   unsigned char *stat;
-  int ngroups;
-  const int *gcount, *flist;
+  int nt;
+  const int *gcount, *flist, *tgfirst;
+  int nsplit;
 };
 
 template <int MAXLMB>
 __global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S, ClassMap cm) {
-  __shared__ double s_r[64], s_w[64], s_cu[64], s_ex[64];
+  __shared__ double s_r[64], s_w[64], s_cu[ECP_MAX_L * 64], s_ex[64];
   __shared__ double s_ka[(MAXLMB + 1) * kKS], s_kb[(MAXLMB + 1) * kKS];
   const int i = class_entry(cm, blockIdx.x);
   const int c = cm.cls[i], bid = blockIdx.x - cm.bbeg[i], nb = cm.bbeg[i + 1] - cm.bbeg[i];
@@ -1539,8 +1577,8 @@ __global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S
 #define X(L)                                                                                                   \
   case L:                                                                                                      \
     if constexpr (L <= MAXLMB)                                                                                 \
-      fb2_body<L>(bid, nb, T, J, S.groups, S.tmpl, S.tf, S.fbq2 + (size_t)L * S.ngroups, S.fbc + 16 + L, S.fbT,   \
-                  S.stat, S.gcount, S.flist, s_r, s_w, s_cu, s_ex, s_ka, s_kb);                                 \
+      fb2_body<L>(bid, nb, T, J, S.groups, S.tmpl, S.tf, S.fbq2 + (size_t)L * S.nt, S.fbc + kFb2Cnt + L, S.fbT, \
+                  S.stat, S.gcount, S.flist, S.tgfirst, S.nsplit, s_r, s_w, s_cu, s_ex, s_ka, s_kb);            \
     break;
     ECP_CASES_LMB(X)
 #undef X
@@ -1934,8 +1972,9 @@ __global__ __launch_bounds__(128) void k_ang(DevTables T, DevJob J, const int2 *
 // first, then device-only buffers
 struct JobLayout {
   size_t i_atoms = 0, i_amaxl = 0, i_sh = 0, i_pa = 0, i_pd = 0, i_cent = 0, i_gauss = 0, i_win = 0, i_trip = 0,
-         i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, in_bytes = 0;
-  size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
+         i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, i_tcls2 = 0,
+         in_bytes = 0;
+  size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_tclaim = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
          d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0, deriv = 0;
 };
@@ -1962,6 +2001,11 @@ struct Prepared {
   unsigned char *t1fb = nullptr, *t2fb = nullptr;  // per triplet: has large-grid work (type 1 / type 2)
   ClassMap cm_head, cm_tail, cm_fb1, cm_fb2;      // fused launches
   int max_lab = 0, max_lmb = 0;
+  int tcount2[kMaxLMB + 1] = {0};  // triplets per type-2 fallback class
+  int max_cls2 = 0;
+  int *tclaim = nullptr;
+  const unsigned char *tcls2 = nullptr;
+  const int *tgf = nullptr;
   int max_lbs = 0, max_L = 0;  // largest basis l and ECP local label of the job
   const T2Group *grp = nullptr;
   const int *goff = nullptr;
@@ -2021,7 +2065,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 15, 100000, 8192, 2, 2, 15, 2048};
+  ecp_tuning tune{8, 15, 100000, 8192, 2, 2, 15, 2048, 0};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph[kSpanGraphs] = {};
   hipGraphExec_t gexec[kSpanGraphs] = {};
@@ -2279,8 +2323,8 @@ namespace {
 void launch_type2(ecp_dev *d, hipStream_t s) {
   const Prepared &P = d->P;
   if (P.ntot2 <= 0) return;
-  T2Args A{P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + 16, P.ngroups,
-           P.t2fb, P.flist, P.tq2, P.fbc + 48};
+  T2Args A{P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + kFb2Cnt, P.nt,
+           P.t2fb, P.flist, P.tq2, P.fbc + 48, P.tclaim, P.tcls2};
   const int head = d->tune.t2_head;
   {
     Span sp(d, s, FAM_TYPE2);
@@ -2347,14 +2391,29 @@ void launch_fb1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, 
   }
 }
 
+// waves per (triplet, pair) of the type-2 fallback (fb2_split; 0: automatic,
+// 3 for jobs of up to kFb2SplitMaxItems (triplet, pair) items, where the
+// fallback's latency counts, 1 for larger, throughput-bound ones)
+constexpr long kFb2SplitMaxItems = 1000000;
+
+int fb2_split(const ecp_dev *d, bool fused) {
+  (void)fused;
+  const int v = d->tune.fb2_split;
+  if (v > 0) return std::min(v, 8);
+  long items = 0;
+  for (int c = 0; c <= kMaxLMB; c++) items += (long)d->P.tcount2[c] * d->P.maxPairs;
+  return items <= kFb2SplitMaxItems ? 3 : 1;
+}
+
 template <int LMB>
 void launch_fb2(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T2Group *g, const int *tm,
                 const unsigned char *tf, const int *fbq, const int *fbc, double *fbT, unsigned char *st,
-                const int *gc, const int *fl, long maxItems) {
+                const int *gc, const int *fl, const int *tgf, long maxItems) {
   if (maxItems > 0) {
     Span sp(d, s, FAM_FB2);
-    const long blocks = std::min<long>(maxItems, 8192);
-    k_fb2<LMB><<<(int)blocks, 64, 0, s>>>(T, J, g, tm, tf, fbq, fbc, fbT, st, gc, fl);
+    const int nsplit = fb2_split(d, false);
+    const long blocks = std::min<long>(maxItems * nsplit, 8192);
+    k_fb2<LMB><<<(int)blocks, 64, 0, s>>>(T, J, g, tm, tf, fbq, fbc, fbT, st, gc, fl, tgf, nsplit);
   }
 }
 
@@ -2430,8 +2489,8 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     }
     (void)hipStreamWaitEvent(s, d->t2done, 0);
 #define FB2(c)                                                                                                  \
-  launch_fb2<c>(d, s, T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.ngroups, P.fbc + 16 + (c), P.fbT,   \
-                P.stat, P.gflag, P.flist, (long)P.lmb_count[c] * P.maxPairs);
+  launch_fb2<c>(d, s, T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.nt, P.fbc + kFb2Cnt + (c), P.fbT,  \
+                P.stat, P.gflag, P.flist, P.tgf, (long)P.tcount2[c] * P.maxPairs);
     FB2(0) FB2(1) FB2(2) FB2(3) FB2(4) FB2(5) FB2(6) FB2(7) FB2(8) FB2(9) FB2(10) FB2(11)
 #undef FB2
     {
@@ -2580,13 +2639,14 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     }
     (void)hipStreamWaitEvent(w[3], d->t2done, 0);
     (void)hipStreamWaitEvent(w[3], d->angdone, 0);
-    T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.ngroups, P.gflag, P.flist};
-    const ClassMap cm2 = class_map(kMaxLMB + 1, P.lmb_count, [&](int c) {
-      return (int)std::min<long>((long)P.lmb_count[c] * P.maxPairs, 8192);
+    const int nsplit = fb2_split(d, true);
+    T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.nt, P.gflag, P.flist, P.tgf, nsplit};
+    const ClassMap cm2 = class_map(kMaxLMB + 1, P.tcount2, [&](int c) {
+      return (int)std::min<long>((long)P.tcount2[c] * P.maxPairs * nsplit, 8192);
     });
     if (cm2.n > 0) {
       Span sp(d, w[3], FAM_FB2);
-      if (P.max_lmb > 7) k_fb2_fused<11><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
+      if (P.max_cls2 > 7) k_fb2_fused<11><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
       else k_fb2_fused<7><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
     }
     {
@@ -2662,11 +2722,20 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   }
   std::vector<T2Group> groups;
   std::vector<int> goff, tgfirst(nt);
+  std::vector<unsigned char> tcls2(nt);
+  for (int c = 0; c <= kMaxLMB; c++) P.tcount2[c] = 0;
+  P.max_cls2 = 0;
   int ntot2 = 0;
   for (int t = 0; t < nt; t++) {
     const ecp_triplet &tr = job->trip[t];
     const int la = tr.la, lb = tr.lb;
     tgfirst[t] = (int)groups.size();
+    {
+      const int c2 = std::max(la, lb) + job->cent[tr.centre].L - 1;
+      tcls2[t] = (unsigned char)c2;
+      P.tcount2[c2]++;
+      P.max_cls2 = std::max(P.max_cls2, c2);
+    }
     for (int l = 0; l < job->cent[tr.centre].L; l++) {
       const int key = tmpl_key(la, lb, l);
       T2Group g{t, l, ntot2, d->tmpl_cnt[key], d->tmpl_off[key], std::max(la, lb) + l};
@@ -2718,6 +2787,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.i_grp = in_take(sizeof(T2Group) * groups.size());
   Ly.i_goff = in_take(4 * goff.size());
   Ly.i_tgf = in_take(4 * (size_t)nt);
+  Ly.i_tcls2 = in_take((size_t)nt);
   Ly.i_fit = in_take(sizeof(int2) * fitems.size());
   Ly.i_ait = in_take(sizeof(int2) * aitems.size());
   Ly.i_tcls = in_take(4 * (size_t)nt);
@@ -2734,6 +2804,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_stat = dv_take(2 * (size_t)nt);
   Ly.d_t1fb = dv_take((size_t)nt);
   Ly.d_t2fb = dv_take((size_t)nt);
+  Ly.d_tclaim = dv_take(4 * (size_t)nt);
   Ly.d_zero_end = dv_off;
   Ly.d_UL = dv_take(8 * job->nUL);
   Ly.d_Ut = dv_take(8 * job->nUt);
@@ -2747,7 +2818,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_tf = dv_take((size_t)ntot2);
   Ly.d_flist = dv_take(4 * (size_t)ntot2);
   Ly.d_tq2 = dv_take(4 * (size_t)ntot2);
-  Ly.d_fbq2 = dv_take(4 * (size_t)(kMaxLMB + 1) * P.ngroups);
+  Ly.d_fbq2 = dv_take(4 * (size_t)(kMaxLMB + 1) * nt);
   Ly.d_fbT = dv_take(8 * (size_t)ntot2 * P.maxPairs);
   Ly.d_blk = dv_take(8 * job->nblk);
   Ly.d_t1s = dv_take(3 * 8 * nq);
@@ -2826,6 +2897,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   cp(Ly.i_grp, groups.data(), sizeof(T2Group) * groups.size());
   cp(Ly.i_goff, goff.data(), 4 * goff.size());
   cp(Ly.i_tgf, tgfirst.data(), 4 * (size_t)nt);
+  cp(Ly.i_tcls2, tcls2.data(), (size_t)nt);
   cp(Ly.i_fit, fitems.data(), sizeof(int2) * fitems.size());
   cp(Ly.i_ait, aitems.data(), sizeof(int2) * aitems.size());
   cp(Ly.i_tcls, tcls.data(), 4 * (size_t)nt);
@@ -2908,6 +2980,9 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.flist = (int *)(A + Ly.d_flist);
   P.tq2 = (int *)(A + Ly.d_tq2);
   P.fbq2 = (int *)(A + Ly.d_fbq2);
+  P.tclaim = (int *)(A + Ly.d_tclaim);
+  P.tcls2 = (const unsigned char *)(A + Ly.i_tcls2);
+  P.tgf = (const int *)(A + Ly.i_tgf);
   P.fbT = (double *)(A + Ly.d_fbT);
   P.fit = (const int2 *)(A + Ly.i_fit);
   P.ait = (const int2 *)(A + Ly.i_ait);

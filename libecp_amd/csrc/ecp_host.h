@@ -84,6 +84,8 @@ typedef struct {
   int t2_head;        /* PS93 levels of the type-2 head kernel before the compacted tail
                          (>= the level count: no tail) */
   int t2_tail_grid;   /* most workgroups of the type-2 tail launch */
+  int fb2_split;      /* waves per (triplet, primitive pair) of the type-2 fallback, its 64-entry
+                         passes dealt round robin; 0: 3 for small jobs, 1 for large ones */
 } ecp_tuning;
 
 /* ---- device layer (ecp_device.hip) ---- */

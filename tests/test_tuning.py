@@ -20,6 +20,8 @@ SETTINGS = [
     {"fuse": 0, "t1_coop_max": 0, "t1_head_serial": 15},         # lane-serial only, no tail
     {"fuse": 0, "t1_coop_max": 0, "t1_head_serial": 4},          # compacted lane-serial tail
     {"fuse": 0, "t1_coop_max": 0, "t1_head_serial": 6, "t1_tail_grid": 3},
+    {"fuse": 1, "fb2_split": 1},                                  # one wave per (triplet, pair)
+    {"fuse": 0, "fb2_split": 4},
 ]
 
 
