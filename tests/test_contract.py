@@ -246,3 +246,49 @@ def test_batch_matrix_refused_for_derivative_batch():
         assert rc == -1 and "derivative" in native.last_error()
     finally:
         b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [0, 1])
+def test_contraction_under_custom_ordering(n):
+    """A custom Cartesian ordering (reversed inside every shell, through
+    libECP_init's lmax / shellOrdering) with the density permuted the same
+    way gives the same energy and gradient: the contraction reads the
+    components (and their shifted partners) through the handle's ordering."""
+    m = plain_small(2)
+    lmax = 10
+    order = []
+    for l in range(lmax + 1):
+        comps = [(l - i, i - j, j) for i in range(l + 1) for j in range(i + 1)]
+        for c in reversed(comps):
+            order.extend(c)
+    ordering = np.asarray(order, dtype=np.int32)
+    perm, o = [], 0
+    for l in m.lBS:
+        nl = (int(l) + 1) * (int(l) + 2) // 2
+        perm.extend(o + nl - 1 - c for c in range(nl))   # new position -> libint position
+        o += nl
+    perm = np.asarray(perm)
+    P = random_density(m.nao, 23)
+    Pr = P[np.ix_(perm, perm)]
+    h0 = native.Handle(m, n=n)
+    h1 = native.Handle(m, n=n, lmax=lmax, ordering=ordering)
+    try:
+        rc0, v0 = h0.contract(P)
+        rc1, v1 = h1.contract(Pr)
+    finally:
+        h0.close()
+        h1.close()
+    assert rc0 == rc1 == 0
+    v0, v1 = np.asarray(v0), np.asarray(v1)
+    assert np.allclose(v0, v1, rtol=1e-12, atol=1e-12 * np.abs(v0).max()), (v0, v1)
+
+
+def test_contract_rejects_short_rows():
+    m = soft_sp()
+    h = native.Handle(m, n=0)
+    try:
+        with pytest.raises(RuntimeError, match="rowdim"):
+            h.contract(np.zeros((m.nao, m.nao - 1)))
+    finally:
+        h.close()
