@@ -1248,17 +1248,32 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
                                          int nsplit, double *s_r, double *s_w, double *s_cu, double *s_ex,
                                          double *s_ka, double *s_kb) {
   const int lane = threadIdx.x;
-  // items (triplet, pair, split): split k takes the passes k, k + nsplit, ...
-  const long total = (long)(*fbcount) * J.maxPairs * nsplit;
+  // items (split, triplet, pair), split-major so that the grid-stride loop
+  // spreads every split over all workgroups; split k takes the passes k,
+  // k + nsplit, ...
+  const long per_split = (long)(*fbcount) * J.maxPairs;
+  const long total = per_split * nsplit;
   for (long it = bid; it < total; it += nb) {
-    const int t = fbq[it / ((long)J.maxPairs * nsplit)];
-    const int rem = (int)(it % ((long)J.maxPairs * nsplit));
-    const int pair = rem / nsplit, split = rem % nsplit;
+    const int split = (int)(it / per_split);
+    const long r = it - split * per_split;
+    const int t = fbq[r / J.maxPairs];
+    const int pair = (int)(r % J.maxPairs);
     const ecp_triplet tr = J.trip[t];
     const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
     if (pair >= Sa.K * Sb.K) continue;
     const int pa = pair / Sb.K, pb = pair % Sb.K;
     const ecp_centre Cn = J.cent[tr.centre];
+    // the triplet's groups (one per semi-local channel l) and their failed
+    // entries; the points are shared by all of them
+    const int g0 = tgfirst[t], L = Cn.L;
+    int gc[ECP_MAX_L], nfail = 0, lmax = 0;
+    for (int l = 0; l < L; l++) {
+      gc[l] = gcount[g0 + l];
+      nfail += gc[l];
+      if (gc[l]) lmax = l;
+    }
+    // this wave's passes: split, split + nsplit, ... (none: nothing to do)
+    if (split * 64 * kFb2PerLane >= nfail) continue;
     const ecp_xyz rc = J.atoms[Cn.atom];
     const double dAC = dist(rc, J.atoms[Sa.atom]), dBC = dist(rc, J.atoms[Sb.atom]);
     const double za = J.pa[Sa.prim + pa], zb = J.pa[Sb.prim + pb];
@@ -1276,15 +1291,6 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     };
     const int brk = fm06_break(T, M, dAC, dBC, minExp, expo_of);
     const int end = brk - 1;
-    // the triplet's groups (one per semi-local channel l) and their failed
-    // entries; the points are shared by all of them
-    const int g0 = tgfirst[t], L = Cn.L;
-    int gc[ECP_MAX_L], nfail = 0, lmax = 0;
-    for (int l = 0; l < L; l++) {
-      gc[l] = gcount[g0 + l];
-      nfail += gc[l];
-      if (gc[l]) lmax = l;
-    }
     const int laC = tr.la + lmax, lbC = tr.lb + lmax;
     const ecp_gauss *gg = J.gauss + Cn.g0;
     auto eval = [&](int slot, int n) {
@@ -1329,7 +1335,6 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     };
     // the failed entries of all groups (listed by k_type2), group after
     // group, 64 per pass
-    if (split * 64 * kFb2PerLane >= nfail) continue;
     for (int pass0 = split * 64 * kFb2PerLane; pass0 < nfail; pass0 += 64 * kFb2PerLane * nsplit) {
       const int nhere = min(64 * kFb2PerLane, nfail - pass0);
       int ent[kFb2PerLane], c1[kFb2PerLane], c2[kFb2PerLane], c3[kFb2PerLane], el[kFb2PerLane];
