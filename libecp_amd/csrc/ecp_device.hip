@@ -1821,8 +1821,8 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
 // NB, NM > 0: the angular factors of atom B for every lambda2 (at most NB
 // values) and m (at most NM) are held in registers for each l, those of atom
 // A per lambda1; NB = 0 reads both from the table in the inner loop.
-template <int NB, int NM>
-__global__ __launch_bounds__(kAsmThreads) void k_asm2(DevTables T, DevJob J, AsmArgs A,
+template <int NB, int NM, int TPB = kAsmThreads>
+__global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
                                                       const unsigned char *__restrict__ status,
                                                       const unsigned char *__restrict__ fbflag, int filter) {
   extern __shared__ double lds[];
@@ -2346,18 +2346,29 @@ void launch_type2(ecp_dev *d, hipStream_t s) {
 // l <= 4 with projectors l <= 4; larger jobs read the tables directly.  The
 // cache costs occupancy: it pays where the assembly is on the critical path
 // of a small job (fused launches), not in large throughput-bound ones.
-void launch_asm2(ecp_dev *d, hipStream_t s, int filter, bool fused) {
+template <int TPB>
+void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache) {
   const Prepared &P = d->P;
+  if (!cache)
+    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  else if (P.max_lbs <= 3 && P.max_L <= 4)
+    k_asm2<4, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  else if (P.max_lbs <= 4 && P.max_L <= 5)
+    k_asm2<5, 9, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  else
+    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+}
+
+// The assembly of the triplets with large-grid work (filter 2) comes last
+// and covers few triplets: with fused launches it runs 4 waves per triplet,
+// so that each triplet's elements finish sooner.
+void launch_asm2(ecp_dev *d, hipStream_t s, int filter, bool fused) {
   const int mode = d->tune.asm2_cache;
   const bool cache = mode == 1 || (mode == 2 && fused);
-  if (!cache)
-    k_asm2<0, 1><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
-  else if (P.max_lbs <= 3 && P.max_L <= 4)
-    k_asm2<4, 7><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
-  else if (P.max_lbs <= 4 && P.max_L <= 5)
-    k_asm2<5, 9><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  if (fused && filter == 2)
+    launch_asm2_tpb<4 * kAsmThreads>(d, s, filter, cache);
   else
-    k_asm2<0, 1><<<P.nt, kAsmThreads, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+    launch_asm2_tpb<kAsmThreads>(d, s, filter, cache);
 }
 
 template <int LAB>
