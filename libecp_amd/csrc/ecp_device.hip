@@ -236,10 +236,11 @@ __device__ void real_sph_inplace(int lmax, double theta, double phi, const doubl
 // different lanes: s1^m by repeated multiplication, the diagonal, the first
 // subdiagonal and the three-term recurrence in l, the normalisation per
 // (l, m), and sin/cos(m phi) by the angle-addition recurrence from m = 1.
-__device__ void real_sph_column(int lmax, int m, double theta, double phi, const double *__restrict__ fac,
-                                const double *__restrict__ dfac, double *out) {
+// x = cos(theta), sp = sin(phi), cp = cos(phi): the direction's trigonometric
+// values, formed once per direction by the caller.
+__device__ void real_sph_column(int lmax, int m, double x, double phi, double sp, double cp,
+                                const double *__restrict__ fac, const double *__restrict__ dfac, double *out) {
 #define RSH(l, mm) ((l) * (l) + (l) + (mm))
-  const double x = cos(theta);
   if (1.0 == x || -1.0 == x) {
     for (int l = m; l <= lmax; l++) {
       out[RSH(l, m)] = 0.0;
@@ -268,7 +269,6 @@ __device__ void real_sph_column(int lmax, int m, double theta, double phi, const
     for (int l = m + 2; l <= lmax; l++)
       out[RSH(l, m)] = (x * (2 * l - 1) * out[RSH(l - 1, m)] - (l + m - 1) * out[RSH(l - 2, m)]) / (l - m);
   }
-  const double sp = sin(phi), cp = cos(phi);
   double sm = 0.0, cm = 1.0;
   for (int k = 1; k <= m; k++) {
     if (0.0 == phi) {
@@ -1612,6 +1612,7 @@ struct AsmArgs {
   // and the type-2 region (blk, T) overlap
   int lds_blk, lds_H, lds_J, lds_rsph, lds_T, lds_rs_stride, pair_chunk, lds_Q;
   int lds_S1, lds_S2, shift_lm;  // shift factor tables (aliasing rsph/Q resp. T) for shells up to shift_lm
+  int lds_dir;                    // type 1: the pair directions of a chunk (4 values each)
 };
 
 // binomial shift of a block over C-centred monomials to the IJK(la) x IJK(lb)
@@ -1743,6 +1744,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
   double *Jbuf = lds + A.lds_J;
   double *rsph = lds + A.lds_rsph;
   double *Qs = lds + A.lds_Q;
+  double *Dir = lds + A.lds_dir;
   const int t = tlist[blockIdx.x];
   // filter 1: only triplets without large-grid work, 2: only those with it
   if ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t])) return;
@@ -1769,17 +1771,28 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
   for (int idx = threadIdx.x; idx < np; idx += blockDim.x) H[idx] = 0.0;
   for (int c0 = 0; c0 < npairs; c0 += A.pair_chunk) {
     const int nc = min(A.pair_chunk, npairs - c0);
-    // harmonics of the chunk's pair directions, one (pair, m) column per lane;
+    // the chunk's pair directions (k of util.c:74-106 and its cos theta,
+    // sin phi, cos phi, as spherical_harmonics.c:15-114 forms them), one
+    // lane per pair, then their harmonics, one (pair, m) column per lane;
     // the pairs' Q values staged alongside
-    for (int it = threadIdx.x; it < nc * (lab + 1); it += blockDim.x) {
-      const int k = it / (lab + 1), m = it - k * (lab + 1);
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
       const T1Unit U = A.units1[u0 + c0 + k];
       const double za = J.pa[Sa.prim + U.pa], zb = J.pa[Sb.prim + U.pb];
       double P[3];
       for (int p = 0; p < 3; p++) P[p] = 2.0 * (za * rAC[p] + zb * rBC[p]);
       double r, th, ph;
       sph_coords(P[0], P[1], P[2], r, th, ph);
-      real_sph_column(lab, m, th, ph, A.fac, A.dfac, rsph + k * A.lds_rs_stride);
+      double *d = Dir + 4 * k;
+      d[0] = cos(th);
+      d[1] = ph;
+      d[2] = sin(ph);
+      d[3] = cos(ph);
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < nc * (lab + 1); it += blockDim.x) {
+      const int k = it / (lab + 1), m = it - k * (lab + 1);
+      const double *d = Dir + 4 * k;
+      real_sph_column(lab, m, d[0], d[1], d[2], d[3], A.fac, A.dfac, rsph + k * A.lds_rs_stride);
     }
     for (int it = threadIdx.x; it < nc * nql; it += blockDim.x) {
       const int k = it / nql, e = it - k * nql;
@@ -2865,6 +2878,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     aa.lds_S1 = o;
     aa.lds_rsph = o; o += aa.pair_chunk * aa.lds_rs_stride;
     aa.lds_Q = o; o += aa.pair_chunk * nq_of(2 * LM);
+    aa.lds_dir = o; o += 4 * aa.pair_chunk;
     const int end1 = std::max(o, aa.lds_S1 + ns);
     // type 2: blk stays for the shift; the shift tables reuse T
     o = base;
