@@ -1595,6 +1595,7 @@ __global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S
 // (calcPolynomials) and norms (libecp.c:234-235)
 
 constexpr int kAsmThreads = 64;
+constexpr int kAsmPack = 4, kAsmPackLab = 2;  // packed type-1 assembly: triplets per wave, largest la + lb
 
 struct AsmArgs {
   const T1Unit *units1;
@@ -1613,6 +1614,7 @@ struct AsmArgs {
   int lds_blk, lds_H, lds_J, lds_rsph, lds_T, lds_rs_stride, pair_chunk, lds_Q;
   int lds_S1, lds_S2, shift_lm;  // shift factor tables (aliasing rsph/Q resp. T) for shells up to shift_lm
   int lds_dir;                    // type 1: the pair directions of a chunk (4 values each)
+  int lds_group;                  // packed type-1 assembly: LDS doubles per triplet
 };
 
 // binomial shift of a block over C-centred monomials to the IJK(la) x IJK(lb)
@@ -1661,10 +1663,11 @@ __device__ __forceinline__ ShiftTab shift_tab_at(double *base, int LM, int side)
   return S;
 }
 
-__device__ void shift_factors(const DevTables &T, int l, const V3 v, bool with_norm, double norm, ShiftTab S) {
+__device__ void shift_factors(const DevTables &T, int l, const V3 v, bool with_norm, double norm, ShiftTab S,
+                              int lid, int nlanes) {
   const int D = T.ijkDim;
   const double tol = T.accuracy;
-  for (int c = threadIdx.x; c < ECP_DIM_IJK(l); c += blockDim.x) {
+  for (int c = lid; c < ECP_DIM_IJK(l); c += nlanes) {
     const int *e = T.ijk + 3 * ECP_IDX_C(l, c);
     const int ax = e[0], ay = e[1], az = e[2];
     int n = 0;
@@ -1697,17 +1700,27 @@ __device__ void shift_factors(const DevTables &T, int l, const V3 v, bool with_n
 // output (util.c:246-334).  The block is read through get(r, q): row r is an
 // ordering position of degree <= la, column q a position of degree <= lb.
 // tab: LDS for the factor tables (shift_tab_doubles(LM) doubles, LM >= la,
-// lb), free to overwrite; Jbuf: LDS for the half-shifted block.
+// lb), free to overwrite; Jbuf: LDS for the half-shifted block.  The lanes
+// lid = 0 .. nlanes - 1 of the workgroup do the work; every lane of the
+// workgroup calls it (it synchronises the workgroup), those of an inactive
+// lane group with active = false.
 template <typename Get>
 __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V3 rA, const V3 rB, double norm,
-                            double *Jbuf, double *tab, int LM, double *out) {
-  const int ncb = ECP_DIM_C(lb), nia = ECP_DIM_IJK(la), nib = ECP_DIM_IJK(lb);
+                            double *Jbuf, double *tab, int LM, double *out, int lid = -1, int nlanes = 0,
+                            bool active = true) {
+  if (lid < 0) {
+    lid = threadIdx.x;
+    nlanes = blockDim.x;
+  }
+  const int ncb = ECP_DIM_C(lb), nia = active ? ECP_DIM_IJK(la) : 0, nib = ECP_DIM_IJK(lb);
   const ShiftTab SA = shift_tab_at(tab, LM, 0), SB = shift_tab_at(tab, LM, 1);
   __syncthreads();
-  shift_factors(T, la, rA, false, 1.0, SA);
-  shift_factors(T, lb, rB, true, norm, SB);
+  if (active) {
+    shift_factors(T, la, rA, false, 1.0, SA, lid, nlanes);
+    shift_factors(T, lb, rB, true, norm, SB, lid, nlanes);
+  }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < nia * ncb; idx += blockDim.x) {
+  for (int idx = lid; idx < nia * ncb; idx += nlanes) {
     const int c1 = idx / ncb, q = idx % ncb;
     const double *f = SA.f + c1 * SA.mt;
     const int *r = SA.r + c1 * SA.mt;
@@ -1717,7 +1730,7 @@ __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V
     Jbuf[idx] = acc;
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < nia * nib; idx += blockDim.x) {
+  for (int idx = lid; idx < nia * nib; idx += nlanes) {
     const int c1 = idx / nib, c2 = idx % nib;
     const double *f = SB.f + c2 * SB.mt;
     const int *r = SB.r + c2 * SB.mt;
@@ -1735,47 +1748,66 @@ __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V
 // Q[deg p][l] with F_pair[p][l] = sum_m S_lm(k^_pair) poly2sph[p][lm] is
 // formed once per monomial (same terms, same order as type1.c:266-295 per
 // element), then shifted to the output block (util.c:246-334).
+// G > 1 (classes la + lb <= 2, where a triplet has at most 10 monomials):
+// G triplets per wave, kAsmThreads / G lanes and an LDS slice of
+// A.lds_group doubles each; the wave's lane groups run the same phases and
+// synchronise together, an inactive group with empty loops.
+template <int G = 1>
 __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, AsmArgs A,
                                                       const unsigned char *__restrict__ status,
-                                                      const int *__restrict__ tlist,
+                                                      const int *__restrict__ tlist, int ntl,
                                                       const unsigned char *__restrict__ fbflag, int filter) {
-  extern __shared__ double lds[];
+  extern __shared__ double lds_all[];
+  constexpr int W = kAsmThreads / G;
+  const int grp = G == 1 ? 0 : (int)threadIdx.x / W;
+  const int lid = G == 1 ? (int)threadIdx.x : (int)threadIdx.x % W;
+  double *lds = lds_all + (size_t)grp * A.lds_group;
   double *H = lds + A.lds_H;
   double *Jbuf = lds + A.lds_J;
   double *rsph = lds + A.lds_rsph;
   double *Qs = lds + A.lds_Q;
   double *Dir = lds + A.lds_dir;
-  const int t = tlist[blockIdx.x];
-  // filter 1: only triplets without large-grid work, 2: only those with it
-  if ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t])) return;
-  if (status[2 * t]) return;
+  int t;
+  bool active = true;
+  if constexpr (G == 1) {
+    t = tlist[blockIdx.x];
+    // filter 1: only triplets without large-grid work, 2: only those with it
+    if ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t])) return;
+    if (status[2 * t]) return;
+  } else {
+    const int ti = blockIdx.x * G + grp;
+    active = ti < ntl;
+    t = active ? tlist[ti] : tlist[0];
+    if (active && ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t]) || status[2 * t])) active = false;
+    if (!wave_any(active)) return;
+  }
   const ecp_triplet tr = J.trip[t];
   const ecp_centre Cn = J.cent[tr.centre];
   const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
   const int la = tr.la, lb = tr.lb, lab = la + lb;  // shifted for derivatives
-  const int nca = ECP_DIM_C(la), ncb = ECP_DIM_C(lb);
   const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
   const double rAC[3] = {ra.x - rc.x, ra.y - rc.y, ra.z - rc.z};
   const double rBC[3] = {rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
   const V3 vA{rAC[0], rAC[1], rAC[2]}, vB{rBC[0], rBC[1], rBC[2]};
   const int D = T.ijkDim;
   const int tmd = ECP_DIM_L(T.tmDim);
-  (void)nca;
-  (void)ncb;
 
-  const int np = ECP_DIM_C(lab);
+  const int np = active ? ECP_DIM_C(lab) : 0;
   const int nql = nq_of(lab);
-  const int npairs = Sa.K * Sb.K;
+  const int npairs = active ? Sa.K * Sb.K : 0;
+  int npmax = npairs;
+  if (G > 1)
+    for (int o = 32; o > 0; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o, 64));
   const int u0 = A.t1first[t];
   int qstart = 0;
-  for (int idx = threadIdx.x; idx < np; idx += blockDim.x) H[idx] = 0.0;
-  for (int c0 = 0; c0 < npairs; c0 += A.pair_chunk) {
-    const int nc = min(A.pair_chunk, npairs - c0);
+  for (int idx = lid; idx < np; idx += W) H[idx] = 0.0;
+  for (int c0 = 0; c0 < npmax; c0 += A.pair_chunk) {
+    const int nc = G == 1 ? min(A.pair_chunk, npairs - c0) : max(0, min(A.pair_chunk, npairs - c0));
     // the chunk's pair directions (k of util.c:74-106 and its cos theta,
     // sin phi, cos phi, as spherical_harmonics.c:15-114 forms them), one
     // lane per pair, then their harmonics, one (pair, m) column per lane;
     // the pairs' Q values staged alongside
-    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+    for (int k = lid; k < nc; k += W) {
       const T1Unit U = A.units1[u0 + c0 + k];
       const double za = J.pa[Sa.prim + U.pa], zb = J.pa[Sb.prim + U.pb];
       double P[3];
@@ -1789,17 +1821,17 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
       d[3] = cos(ph);
     }
     __syncthreads();
-    for (int it = threadIdx.x; it < nc * (lab + 1); it += blockDim.x) {
+    for (int it = lid; it < nc * (lab + 1); it += W) {
       const int k = it / (lab + 1), m = it - k * (lab + 1);
       const double *d = Dir + 4 * k;
       real_sph_column(lab, m, d[0], d[1], d[2], d[3], A.fac, A.dfac, rsph + k * A.lds_rs_stride);
     }
-    for (int it = threadIdx.x; it < nc * nql; it += blockDim.x) {
+    for (int it = lid; it < nc * nql; it += W) {
       const int k = it / nql, e = it - k * nql;
       Qs[k * nql + e] = A.qout[A.units1[u0 + c0 + k].qoff + e];
     }
     __syncthreads();
-    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    for (int p = lid; p < np; p += W) {
       const int *e = T.ijk + 3 * p;
       const int deg = e[0] + e[1] + e[2];
       qstart = 0;
@@ -1824,7 +1856,11 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
     const int *ea = T.ijk + 3 * r, *eb = T.ijk + 3 * q;
     return H[T.ijkIndex[((ea[0] + eb[0]) * D + ea[1] + eb[1]) * D + ea[2] + eb[2]]];
   };
-  shift_block(T, get1, tr.oa, tr.ob, vA, vB, norm1, Jbuf, lds + A.lds_S1, A.shift_lm, A.blocks + tr.blk1);
+  if constexpr (G == 1)
+    shift_block(T, get1, tr.oa, tr.ob, vA, vB, norm1, Jbuf, lds + A.lds_S1, A.shift_lm, A.blocks + tr.blk1);
+  else
+    shift_block(T, get1, tr.oa, tr.ob, vA, vB, norm1, Jbuf, lds + A.lds_S1, A.shift_lm, A.blocks + tr.blk1, lid, W,
+                active);
 }
 
 // Type-2 block of one triplet per wave: for each projector l, the T_l
@@ -2035,10 +2071,11 @@ struct Prepared {
   double *fbT = nullptr;
   const int2 *fit = nullptr, *ait = nullptr;
   AsmArgs aa{};
+  AsmArgs aa4{};  // packed type-1 assembly of classes la + lb <= kAsmPackLab
   unsigned int *zero_base = nullptr;
   size_t zero_words = 0;
   double *blk = nullptr;
-  size_t asm1_lds = 0, asm2_lds = 0;
+  size_t asm1_lds = 0, asm2_lds = 0, asm1_lds4 = 0;
   int tcls_begin[ECP_MAX_LAB + 2] = {0};  // triplets of each class lambda = la + lb in tcls
   const int *tcls = nullptr;
   // algorithmic FLOPs per evaluation (the per-unit models of the reference's
@@ -2372,6 +2409,20 @@ void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache) {
     k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
 }
 
+// type-1 assembly of the triplets tcls[b .. e) (class-ordered), one per
+// wave, or with pack those of classes la + lb <= kAsmPackLab kAsmPack to a
+// wave (one more launch: it pays in large per-class launches, not on the
+// critical chain of fused ones)
+void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack) {
+  const Prepared &P = d->P;
+  const int m = pack ? std::min(std::max(b, P.tcls_begin[kAsmPackLab + 1]), e) : b;
+  if (m > b)
+    k_asm1<kAsmPack><<<(m - b + kAsmPack - 1) / kAsmPack, kAsmThreads, P.asm1_lds4, s>>>(
+        d->T, P.J, P.aa4, P.stat, P.tcls + b, m - b, P.t1fb, filter);
+  if (e > m)
+    k_asm1<1><<<e - m, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, P.aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
+}
+
 // The assembly of the triplets with large-grid work (filter 2) comes last
 // and covers few triplets: with fused launches it runs 4 waves per triplet,
 // so that each triplet's elements finish sooner.
@@ -2539,8 +2590,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
                   P.fbq1 + P.cls_begin[c], P.fbc + c, P.stat);                                                  \
     if (P.tcls_begin[(c) + 1] > P.tcls_begin[c]) {                                                            \
       Span sp(d, ws, FAM_ASM);                                                                                  \
-      k_asm1<<<P.tcls_begin[(c) + 1] - P.tcls_begin[c], kAsmThreads, P.asm1_lds, ws>>>(                          \
-          T, J, P.aa, P.stat, P.tcls + P.tcls_begin[c], P.t1fb, 0);                                             \
+      launch_asm1(d, ws, P.tcls_begin[c], P.tcls_begin[(c) + 1], 0, true);                                      \
     }                                                                                                           \
   }
   T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10)
@@ -2639,7 +2689,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     (void)hipEventRecord(d->t1done, w[0]);
     {
       Span sp(d, w[0], FAM_ASM);
-      k_asm1<<<P.nt, kAsmThreads, P.asm1_lds, w[0]>>>(T, J, P.aa, P.stat, P.tcls, P.t1fb, 1);
+      launch_asm1(d, w[0], 0, P.nt, 1, false);
     }
     (void)hipStreamWaitEvent(w[1], d->t1done, 0);
     const ClassMap cmf = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return std::min(P.cls_count[c], 4096); });
@@ -2650,7 +2700,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     }
     {
       Span sp(d, w[1], FAM_ASM);
-      k_asm1<<<P.nt, kAsmThreads, P.asm1_lds, w[1]>>>(T, J, P.aa, P.stat, P.tcls, P.t1fb, 2);
+      launch_asm1(d, w[1], 0, P.nt, 2, false);
     }
   }
   // ---- type 2 ----
@@ -2887,6 +2937,23 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     aa.lds_T = o; o += (LM + LC) * (LM + LC) * (2 * LM + 1);
     const int end2 = std::max(o, aa.lds_S2 + ns);
     P.asm1_lds = (size_t)end1 * sizeof(double);
+    // packed type 1 (la + lb <= kAsmPackLab): one slice of LDS per triplet
+    {
+      AsmArgs &a4 = P.aa4;
+      const int LS = std::min(LM, kAsmPackLab), LQ = std::min(2 * LM, kAsmPackLab);
+      a4.pair_chunk = aa.pair_chunk;
+      a4.lds_rs_stride = (LQ + 1) * (LQ + 1);
+      a4.shift_lm = LS;
+      int o4 = 0;
+      a4.lds_J = o4; o4 += ECP_DIM_IJK(LS) * ECP_DIM_C(LS);
+      a4.lds_H = o4; o4 += ECP_DIM_C(LQ);
+      a4.lds_S1 = o4;
+      a4.lds_rsph = o4; o4 += a4.pair_chunk * a4.lds_rs_stride;
+      a4.lds_Q = o4; o4 += a4.pair_chunk * nq_of(LQ);
+      a4.lds_dir = o4; o4 += 4 * a4.pair_chunk;
+      a4.lds_group = std::max(o4, a4.lds_S1 + shift_tab_doubles(LS));
+      P.asm1_lds4 = (size_t)kAsmPack * a4.lds_group * sizeof(double);
+    }
     P.asm2_lds = (size_t)end2 * sizeof(double);
     if (std::max(P.asm1_lds, P.asm2_lds) > 64 * 1024) {
       snprintf(err, errlen, "assembly needs %zu bytes of LDS (l too high for this build)",
@@ -3020,19 +3087,21 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.blk = (double *)(A + Ly.d_blk);
   P.zero_base = (unsigned int *)(A + Ly.d_zero);
   P.zero_words = (Ly.d_zero_end - Ly.d_zero) / 4;
-  AsmArgs &aa = P.aa;
-  aa.units1 = P.u1;
-  aa.t1first = (const int *)(A + Ly.i_t1f);
-  aa.qout = P.q;
-  aa.groups = P.grp;
-  aa.tgfirst = (const int *)(A + Ly.i_tgf);
-  aa.tmpl = d->tmpl_dev;
-  aa.Tout = P.Tout;
-  aa.tfail = P.tf;
-  aa.fbT = P.fbT;
-  aa.fac = d->fac_dev;
-  aa.dfac = d->dfac_dev;
-  aa.blocks = P.blk;
+  for (AsmArgs *x : {&P.aa, &P.aa4}) {
+    AsmArgs &aa = *x;
+    aa.units1 = P.u1;
+    aa.t1first = (const int *)(A + Ly.i_t1f);
+    aa.qout = P.q;
+    aa.groups = P.grp;
+    aa.tgfirst = (const int *)(A + Ly.i_tgf);
+    aa.tmpl = d->tmpl_dev;
+    aa.Tout = P.Tout;
+    aa.tfail = P.tf;
+    aa.fbT = P.fbT;
+    aa.fac = d->fac_dev;
+    aa.dfac = d->dfac_dev;
+    aa.blocks = P.blk;
+  }
   d->prepared = true;
   return 0;
 }
