@@ -631,6 +631,30 @@ __global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Uni
 // Wave-level helpers (blocks of exactly one 64-lane wave).
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// I + v[0] + v[1] + ... + v[n - 1] for v in LDS, added left to right (the
+// order of a lane-serial integrator); the reads are issued eight (then
+// four) at a time so that one wait covers several of them
+__device__ __forceinline__ double lds_sum_in_order(double I, const double *v, int n) {
+  int k = 0;
+  for (; k + 8 <= n; k += 8) {
+    double x[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = v[k + i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) I += x[i];
+  }
+  if (k + 4 <= n) {
+    double x[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) x[i] = v[k + i];
+#pragma unroll
+    for (int i = 0; i < 4; i++) I += x[i];
+    k += 4;
+  }
+  for (; k < n; k++) I += v[k];
+  return I;
+}
+
 // Tail: one wave per primitive pair left open by the head, levels [head,
 // nlevels).  Lane k evaluates the k-th point pair of a round of levels (at
 // most 64 pairs per round; a level of the 383-point grid has at most 64) for
@@ -723,8 +747,7 @@ __device__ __forceinline__ void t1tail_body(int bid, int nb, const DevTables &T,
         const int o = T.s93.lvl_begin[l] - b0, np = T.s93.lvl_begin[l + 1] - T.s93.lvl_begin[l];
         const unsigned long long lm = (np >= 64) ? ~0ull : (((1ull << np) - 1ull) << o);
         const bool any = (cmask & lm) != 0ull;
-        if (lane < NQ)
-          for (int k = 0; k < np; k++) I += s_T[lane * kTS + o + k];
+        if (lane < NQ) I = lds_sum_in_order(I, s_T + lane * kTS + o, np);
         if (!mydone) {
           const int j = T.s93.lvl_j[l], nn = T.s93.lvl_n[l];
           double err;
@@ -952,8 +975,7 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
         s_t[j * kFb1TS + k] = t;
       }
       __syncthreads();
-      if (mine && !done)
-        for (int k = 0; k < np; k++) I += s_t[lane * kFb1TS + k];
+      if (mine && !done) I = lds_sum_in_order(I, s_t + lane * kFb1TS, np);
       __syncthreads();
     };
     int lv = 0;
