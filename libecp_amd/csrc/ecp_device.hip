@@ -1405,12 +1405,26 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
           }
         }
       };
+      // pair sums (0 + f(left)) + f(right) added to I in pair order; four
+      // pairs' sums are formed before they are added, so that their LDS
+      // reads overlap
       auto sum = [&](int l, int slot0, int kb, int ke) {
         const int b0 = T.s92.lvl_begin[l];
 #pragma unroll
         for (int j = 0; j < kFb2PerLane; j++) {
           if (done & (1u << j)) continue;
-          for (int k = kb; k < ke; k++) {
+          int k = kb;
+          for (; k + 4 <= ke; k += 4) {
+            double t[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              t[i] = f(j, slot0 + 2 * (k + i - kb));
+              if (T.p92r[b0 + k + i] <= end) t[i] += f(j, slot0 + 2 * (k + i - kb) + 1);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) I[j] += t[i];
+          }
+          for (; k < ke; k++) {
             double t = f(j, slot0 + 2 * (k - kb));
             if (T.p92r[b0 + k] <= end) t += f(j, slot0 + 2 * (k - kb) + 1);
             I[j] += t;
