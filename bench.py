@@ -449,7 +449,13 @@ def main():
             single, multi = cpu
             out["cpu_baseline"] = multi
             out["cpu_baseline_single_core"] = single
-            out["max_abs_vs_reference"] = float(np.abs(I_gpu - I_ref).max())
+            d = np.abs(I_gpu - I_ref)
+            at = int(np.argmax(d))
+            out["max_abs_vs_reference"] = float(d.flat[at])
+            out["reference_at_max_abs"] = float(I_ref.flat[at])
+            out["elements_above_1e-10_abs"] = int((d > 1e-10).sum())
+            # <= 0: every element within 1e-10 absolute + 1e-8 relative
+            out["max_excess_over_tolerance"] = float((d - (1e-10 + 1e-8 * np.abs(I_ref))).max())
             out["speedup_vs_cpu_all_cores"] = value / multi["value"]
 
     print(json.dumps(out))
