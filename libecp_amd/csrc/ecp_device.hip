@@ -2722,17 +2722,20 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     }
     (void)hipEventRecordWithFlags(d->span[d->span_k][1], w[0], d->capturing ? hipEventRecordExternal : 0);
     d->span_valid = true;
+    // the fallback is captured before the assembly beside it: a graph
+    // continues a node's queue with its first dependent, and the fallback
+    // is the critical one
     (void)hipEventRecord(d->t1done, w[0]);
-    {
-      Span sp(d, w[0], FAM_ASM);
-      launch_asm1(d, w[0], 0, P.nt, 1, false);
-    }
     (void)hipStreamWaitEvent(w[1], d->t1done, 0);
     const ClassMap cmf = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return std::min(P.cls_count[c], 4096); });
     {
       Span sp(d, w[1], FAM_FB1);
       if (big1) k_fb1_fused<10><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
       else k_fb1_fused<6><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+    }
+    {
+      Span sp(d, w[0], FAM_ASM);
+      launch_asm1(d, w[0], 0, P.nt, 1, false);
     }
     {
       Span sp(d, w[1], FAM_ASM);
@@ -2745,15 +2748,11 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     launch_ang(d, s);
   }
   if (P.ntot2 > 0) {
+    // likewise the fallback first (it needs no angular tables), then the
+    // assembly beside it, then the assembly that waits for the fallback
     launch_type2(d, w[2]);
     (void)hipEventRecord(d->t2done, w[2]);
-    (void)hipStreamWaitEvent(w[2], d->angdone, 0);
-    {
-      Span sp(d, w[2], FAM_ASM);
-      launch_asm2(d, w[2], 1, true);
-    }
     (void)hipStreamWaitEvent(w[3], d->t2done, 0);
-    (void)hipStreamWaitEvent(w[3], d->angdone, 0);
     const int nsplit = fb2_split(d, true);
     T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.nt, P.gflag, P.flist, P.tgf, nsplit};
     const ClassMap cm2 = class_map(kMaxLMB + 1, P.tcount2, [&](int c) {
@@ -2764,6 +2763,12 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       if (P.max_cls2 > 7) k_fb2_fused<11><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
       else k_fb2_fused<7><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
     }
+    (void)hipStreamWaitEvent(w[2], d->angdone, 0);
+    {
+      Span sp(d, w[2], FAM_ASM);
+      launch_asm2(d, w[2], 1, true);
+    }
+    (void)hipStreamWaitEvent(w[3], d->angdone, 0);
     {
       Span sp(d, w[3], FAM_ASM);
       launch_asm2(d, w[3], 2, true);
