@@ -628,13 +628,32 @@ __global__ __launch_bounds__(64) void k_type1(DevTables T, DevJob J, const T1Uni
                         tq, tcount, fbflag);
 }
 
+// f + a[0] b[0] + a[sa] b[sb] + ... (n terms), accumulated in order; the
+// reads of up to B terms are issued before their products are added
+template <int B>
+__device__ __forceinline__ double dot_in_order(double f, const double *a, int sa, const double *b, int sb, int n) {
+  int m = 0;
+  for (; m + B <= n; m += B) {
+    double x[B], y[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+      x[i] = a[(m + i) * sa];
+      y[i] = b[(m + i) * sb];
+    }
+#pragma unroll
+    for (int i = 0; i < B; i++) f += x[i] * y[i];
+  }
+  for (; m < n; m++) f += a[m * sa] * b[m * sb];
+  return f;
+}
+
 // Wave-level helpers (blocks of exactly one 64-lane wave).
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
-// I + v[0] + v[1] + ... + v[n - 1] for v in LDS, added left to right (the
-// order of a lane-serial integrator); the reads are issued eight (then
-// four) at a time so that one wait covers several of them
-__device__ __forceinline__ double lds_sum_in_order(double I, const double *v, int n) {
+// I + v[0] + v[1] + ... + v[n - 1], added left to right (the order of a
+// lane-serial integrator); the reads (LDS or global) are issued eight
+// (then four) at a time so that one wait covers several of them
+__device__ __forceinline__ double sum_in_order(double I, const double *v, int n) {
   int k = 0;
   for (; k + 8 <= n; k += 8) {
     double x[8];
@@ -747,7 +766,7 @@ __device__ __forceinline__ void t1tail_body(int bid, int nb, const DevTables &T,
         const int o = T.s93.lvl_begin[l] - b0, np = T.s93.lvl_begin[l + 1] - T.s93.lvl_begin[l];
         const unsigned long long lm = (np >= 64) ? ~0ull : (((1ull << np) - 1ull) << o);
         const bool any = (cmask & lm) != 0ull;
-        if (lane < NQ) I = lds_sum_in_order(I, s_T + lane * kTS + o, np);
+        if (lane < NQ) I = sum_in_order(I, s_T + lane * kTS + o, np);
         if (!mydone) {
           const int j = T.s93.lvl_j[l], nn = T.s93.lvl_n[l];
           double err;
@@ -975,7 +994,7 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
         s_t[j * kFb1TS + k] = t;
       }
       __syncthreads();
-      if (mine && !done) I = lds_sum_in_order(I, s_t + lane * kFb1TS, np);
+      if (mine && !done) I = sum_in_order(I, s_t + lane * kFb1TS, np);
       __syncthreads();
     };
     int lv = 0;
@@ -1879,7 +1898,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
         const double *Q = Qs + k * nql + qstart;
         for (int l = deg; l >= 0; l -= 2) {
           double f = 0.0;
-          for (int m = 0; m < 2 * l + 1; m++) f += rs[ECP_IDX_LM(l, m)] * pm[ECP_IDX_LM(l, m)];
+          f = dot_in_order<4>(f, rs + ECP_IDX_LM(l, 0), 1, pm + ECP_IDX_LM(l, 0), 1, 2 * l + 1);
           acc += f * Q[(deg - l) / 2];
         }
       }
@@ -1947,9 +1966,7 @@ __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
       if (!A.tfail[G.eoff + e]) {
         v = A.Tout[G.eoff + e];
       } else {
-        v = 0.0;
-        const double *fb = A.fbT + (size_t)(G.eoff + e) * J.maxPairs;
-        for (int pr = 0; pr < Sa.K * Sb.K; pr++) v += fb[pr];
+        v = sum_in_order(0.0, A.fbT + (size_t)(G.eoff + e) * J.maxPairs, Sa.K * Sb.K);
       }
       Tl[(l1 * d2 + l2) * d3 + l3] = v;
     }
@@ -1992,9 +2009,9 @@ __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
       } else {
         for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
           for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
-            double factor = 0.0;
             const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
             const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
+            double factor = 0.0;
             for (int m = 0; m < nm; m++) factor += pa[m * J.ang_c] * pb[m * J.ang_c];
             tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
           }
