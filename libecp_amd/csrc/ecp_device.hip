@@ -1898,7 +1898,7 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
         const double *Q = Qs + k * nql + qstart;
         for (int l = deg; l >= 0; l -= 2) {
           double f = 0.0;
-          f = dot_in_order<4>(f, rs + ECP_IDX_LM(l, 0), 1, pm + ECP_IDX_LM(l, 0), 1, 2 * l + 1);
+          f = dot_in_order<2>(f, rs + ECP_IDX_LM(l, 0), 1, pm + ECP_IDX_LM(l, 0), 1, 2 * l + 1);
           acc += f * Q[(deg - l) / 2];
         }
       }
