@@ -105,6 +105,10 @@ int libecp_load_xyz(const char *path, int *nrAtoms, double **geometry) {
   }
   r.line++;
   double *g = malloc(sizeof(double) * 3 * (size_t)nat);
+  if (!g) {
+    fclose(r.fp);
+    return fail(&r, "out of memory");
+  }
   for (int i = 0; i < nat; i++) {
     if (next_line(&r)) {
       free(g);
@@ -131,28 +135,42 @@ int libecp_load_xyz(const char *path, int *nrAtoms, double **geometry) {
   return 0;
 }
 
-/* growable arrays */
+/* growable arrays; a failed allocation sets oom and drops the value */
 typedef struct {
   int *v;
   size_t n, cap;
+  int oom;
 } ivec;
 typedef struct {
   double *v;
   size_t n, cap;
+  int oom;
 } dvec;
 
 static void ipush(ivec *a, int x) {
   if (a->n == a->cap) {
-    a->cap = a->cap ? 2 * a->cap : 16;
-    a->v = realloc(a->v, sizeof(int) * a->cap);
+    const size_t cap = a->cap ? 2 * a->cap : 16;
+    int *v = realloc(a->v, sizeof(int) * cap);
+    if (!v) {
+      a->oom = 1;
+      return;
+    }
+    a->v = v;
+    a->cap = cap;
   }
   a->v[a->n++] = x;
 }
 
 static void dpush(dvec *a, double x) {
   if (a->n == a->cap) {
-    a->cap = a->cap ? 2 * a->cap : 16;
-    a->v = realloc(a->v, sizeof(double) * a->cap);
+    const size_t cap = a->cap ? 2 * a->cap : 16;
+    double *v = realloc(a->v, sizeof(double) * cap);
+    if (!v) {
+      a->oom = 1;
+      return;
+    }
+    a->v = v;
+    a->cap = cap;
   }
   a->v[a->n++] = x;
 }
@@ -199,6 +217,7 @@ int libecp_load_basis(const char *path, int nrAtoms, int **shellsBS, int **lBS, 
     }
   }
   rc = (int)l.n;
+  if (sh.oom || l.oom || K.oom || a.oom || d.oom) rc = fail(&r, "out of memory");
 out:
   fclose(r.fp);
   if (rc < 0) {
@@ -253,6 +272,7 @@ int libecp_load_ecp(const char *path, int nrAtoms, int **shellsECP, int **lECP, 
     }
   }
   rc = (int)l.n;
+  if (sh.oom || l.oom || K.oom || core.oom || a.oom || d.oom || pw.oom) rc = fail(&r, "out of memory");
 out:
   fclose(r.fp);
   if (rc < 0) {
