@@ -82,6 +82,10 @@ libECPHandle *libECP_init(int nrAtoms, double *geometry, int *shellsECP, int *lE
     return NULL;
   }
   libECPHandle *h = calloc(1, sizeof(*h));
+  if (!h) {
+    set_last_error("out of host memory");
+    return NULL;
+  }
   h->nat = nrAtoms;
   h->n = n;
   ecp_mol m0 = {nrAtoms, geometry, shellsECP, lECP, KECP, nECP, dECP, aECP, shellsBS, lBS, KBS, dBS, aBS};
@@ -152,12 +156,16 @@ static int run_handle(libECPHandle *h) {
   if (h->job.nblk > h->cap_blocks) {
     free(h->blocks);
     h->blocks = malloc(sizeof(double) * h->job.nblk);
-    h->cap_blocks = h->job.nblk;
+    h->cap_blocks = h->blocks ? h->job.nblk : 0;
   }
   if (2 * (size_t)h->job.ntrip > h->cap_status) {
     free(h->status);
     h->status = malloc(2 * (size_t)h->job.ntrip);
-    h->cap_status = 2 * (size_t)h->job.ntrip;
+    h->cap_status = h->status ? 2 * (size_t)h->job.ntrip : 0;
+  }
+  if ((h->job.nblk && !h->blocks) || (h->job.ntrip && !h->status)) {
+    set_last_error("out of host memory for the blocks");
+    return -1;
   }
   if (h->job.ntrip == 0) return 0;
   if (ensure_device(h)) return -1;
@@ -209,6 +217,11 @@ int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *
   }
   if (h->job.ntrip > 0) {
     int *aoff = malloc(sizeof(int) * ((size_t)h->job.nsh + 1));
+    if (!aoff) {
+      fprintf(stderr, "libecp: out of host memory\n");
+      libECP_free(h);
+      return 1;
+    }
     ecp_job_ao_offsets(&h->job, aoff);
     /* the reference stops accumulating at a failed integration */
     (void)ecp_job_scatter(&h->job, 0, aoff, h->blocks, h->status, rowdim, 0, I);
@@ -352,11 +365,20 @@ libecp_batch *libecp_batch_create_n(const libecp_molecule *mols, int nmol, int n
     return NULL;
   }
   libecp_batch *b = calloc(1, sizeof(*b));
-  b->nmol = nmol;
+  if (!b) {
+    set_last_error("out of host memory");
+    return NULL;
+  }
   b->n = n;
   b->nao = calloc((size_t)nmol, sizeof(int));
   b->nat = calloc((size_t)nmol, sizeof(int));
   b->mols = calloc((size_t)nmol, sizeof(ecp_mol));
+  if (!b->nao || !b->nat || !b->mols) {
+    set_last_error("out of host memory");
+    libecp_batch_free(b);
+    return NULL;
+  }
+  b->nmol = nmol;
   int maxL = 1, maxLBS = 0;
   for (int i = 0; i < nmol; i++) {
     const libecp_molecule *M = &mols[i];
@@ -383,9 +405,14 @@ libecp_batch *libecp_batch_create_n(const libecp_molecule *mols, int nmol, int n
     return NULL;
   }
   b->aoff = malloc(sizeof(int) * ((size_t)b->job.nsh + 1));
-  ecp_job_ao_offsets(&b->job, b->aoff);
   b->blocks = malloc(sizeof(double) * (b->job.nblk + 1));
   b->status = calloc(2 * (size_t)b->job.ntrip + 1, 1);
+  if (!b->aoff || !b->blocks || !b->status) {
+    set_last_error("out of host memory for the batch's blocks");
+    libecp_batch_free(b);
+    return NULL;
+  }
+  ecp_job_ao_offsets(&b->job, b->aoff);
   b->dev = ecp_dev_create(&b->st, err, sizeof(err));
   if (!b->dev || ecp_dev_prepare(b->dev, &b->job, err, sizeof(err))) {
     set_last_error(err);
@@ -437,6 +464,13 @@ int libecp_batch_contract(libecp_batch *b, const double *P, double *out, int *co
   long long *poff = malloc(sizeof(long long) * (size_t)b->nmol);
   int *prow = malloc(sizeof(int) * (size_t)b->nmol);
   int *cd = codes ? codes : malloc(sizeof(int) * (size_t)b->nmol);
+  if (!poff || !prow || !cd) {
+    set_last_error("out of host memory");
+    free(poff);
+    free(prow);
+    if (cd != codes) free(cd);
+    return -1;
+  }
   size_t nP = 0;
   for (int m = 0; m < b->nmol; m++) {
     poff[m] = (long long)nP;

@@ -241,6 +241,10 @@ int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st
                   size_t errlen) {
   memset(job, 0, sizeof(*job));
   ecp_mol_dims *dims = calloc((size_t)nmol, sizeof(ecp_mol_dims));
+  if (!dims) {
+    snprintf(err, errlen, "out of host memory");
+    return -1;
+  }
   int nat = 0, nsh = 0, nprim = 0, ncent = 0, ngauss = 0, nwin = 0;
   int maxLBS = 0, maxL = 0;
   for (int m = 0; m < nmol; m++) {
@@ -299,6 +303,14 @@ int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st
   job->trip_mol_begin = calloc((size_t)nmol + 1, sizeof(int));
   job->mol_atom0 = calloc((size_t)nmol + 1, sizeof(int));
   double *radius = calloc((size_t)nsh + 1, sizeof(double));
+  if (!job->atoms || !job->atom_maxl || !job->atom_shell0 || !job->atom_nsh || !job->shells || !job->pa ||
+      !job->pd || !job->cent || !job->gauss || !job->win || !job->trip_mol_begin || !job->mol_atom0 || !radius) {
+    snprintf(err, errlen, "out of host memory");
+    free(radius);
+    free(dims);
+    ecp_job_free(job);
+    return -1;
+  }
   trip_vec tv = {0};
   int a0 = 0, s0 = 0, p0 = 0, c0 = 0, g0 = 0, w0 = 0;
   size_t blk = 0, ul = 0, ut = 0, fo = 0, ango = 0;

@@ -257,13 +257,18 @@ static void *worker_main(void *arg) {
         free(w->blocks);
         w->cap_blocks = job.nblk + job.nblk / 4 + 1;
         w->blocks = malloc(sizeof(double) * w->cap_blocks);
+        if (!w->blocks) w->cap_blocks = 0;
       }
       if (2 * (size_t)job.ntrip + 1 > w->cap_status) {
         free(w->status);
         w->cap_status = 2 * (size_t)job.ntrip + (size_t)job.ntrip / 2 + 1;
         w->status = malloc(w->cap_status);
+        if (!w->status) w->cap_status = 0;
       }
-      if (ecp_dev_bind(w->dev, s->mem.base + off, bytes, err, sizeof(err)) ||
+      if (!w->blocks || !w->status) {
+        snprintf(err, sizeof(err), "out of host memory for the blocks");
+        dev_rc = -1;
+      } else if (ecp_dev_bind(w->dev, s->mem.base + off, bytes, err, sizeof(err)) ||
           ecp_dev_execute(w->dev, 1, NULL, err, sizeof(err)) ||
           ecp_dev_fetch(w->dev, w->blocks, w->status, err, sizeof(err)))
         dev_rc = -1;
@@ -276,7 +281,11 @@ static void *worker_main(void *arg) {
 
     /* per-request matrices (host) */
     int *aoff = malloc(sizeof(int) * ((size_t)job.nsh + 1));
-    ecp_job_ao_offsets(&job, aoff);
+    if (!aoff && !dev_rc) {
+      snprintf(err, sizeof(err), "out of host memory");
+      dev_rc = -1;
+    }
+    if (aoff) ecp_job_ao_offsets(&job, aoff);
     int rcs[n > 0 ? n : 1];
     for (int i = 0; i < n; i++) {
       req *r = rs[i];
