@@ -1797,6 +1797,38 @@ __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V
   __syncthreads();
 }
 
+// H[p] += sum over the chunk's pairs k, in pair order, of
+// sum_{l = D, D - 2, ..} (sum_m S_lm(k) poly2sph[p][lm]) Q_k[(D - l) / 2]
+// for a monomial p of degree D: the same terms in the same order as the
+// general loop in k_asm1, with the monomial's poly2sph row held in
+// registers across the pairs instead of read again for every pair.
+template <int D>
+__device__ __forceinline__ double asm1_monomial(double acc, const double *__restrict__ pm, const double *rsph,
+                                                int rs_stride, const double *Qs, int nql, int nc) {
+  constexpr int NPM = (D + 1) * (D + 2) / 2;  // sum of 2 l + 1 over l = D, D - 2, ..
+  double pmv[NPM];
+  {
+    int j = 0;
+#pragma unroll
+    for (int l = D; l >= 0; l -= 2)
+#pragma unroll
+      for (int m = 0; m < 2 * l + 1; m++) pmv[j++] = pm[ECP_IDX_LM(l, m)];
+  }
+  for (int k = 0; k < nc; k++) {
+    const double *rs = rsph + k * rs_stride;
+    const double *Q = Qs + k * nql;
+    int j = 0;
+#pragma unroll
+    for (int l = D; l >= 0; l -= 2) {
+      double f = 0.0;
+#pragma unroll
+      for (int m = 0; m < 2 * l + 1; m++) f += rs[ECP_IDX_LM(l, m)] * pmv[j++];
+      acc += f * Q[(D - l) / 2];
+    }
+  }
+  return acc;
+}
+
 // Type-1 block of one triplet per wave (the triplets of one class
 // lambda = la + lb, listed in tlist).  chi[i][j] depends on (i, j) only
 // through the monomial p = i + j, so H[p] = sum_pairs sum_l F_pair[p][l]
@@ -1808,7 +1840,7 @@ __device__ void shift_block(const DevTables &T, Get get, int la, int lb, const V
 // A.lds_group doubles each; the wave's lane groups run the same phases and
 // synchronise together, an inactive group with empty loops.
 template <int G = 1>
-__global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, AsmArgs A,
+__global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, AsmArgs A,
                                                       const unsigned char *__restrict__ status,
                                                       const int *__restrict__ tlist, int ntl,
                                                       const unsigned char *__restrict__ fbflag, int filter) {
@@ -1893,6 +1925,16 @@ __global__ __launch_bounds__(kAsmThreads, 4) void k_asm1(DevTables T, DevJob J, 
       for (int l1 = 0; l1 < deg; l1++) qstart += l1 / 2 + 1;
       const double *pm = T.poly2sph + (size_t)p * tmd;
       double acc = H[p];
+      if (deg <= 6) {
+        switch (deg) {
+#define MONO(D) \
+  case D: acc = asm1_monomial<D>(acc, pm, rsph, A.lds_rs_stride, Qs + qstart, nql, nc); break;
+          MONO(0) MONO(1) MONO(2) MONO(3) MONO(4) MONO(5) MONO(6)
+#undef MONO
+        }
+        H[p] = acc;
+        continue;
+      }
       for (int k = 0; k < nc; k++) {
         const double *rs = rsph + k * A.lds_rs_stride;
         const double *Q = Qs + k * nql + qstart;
