@@ -2053,8 +2053,7 @@ __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
           for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
             const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
             const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
-            double factor = 0.0;
-            for (int m = 0; m < nm; m++) factor += pa[m * J.ang_c] * pb[m * J.ang_c];
+            const double factor = dot_in_order<2>(0.0, pa, J.ang_c, pb, J.ang_c, nm);
             tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
           }
       }
