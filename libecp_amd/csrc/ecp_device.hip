@@ -1829,8 +1829,8 @@ __device__ __forceinline__ double asm1_monomial(double acc, const double *__rest
   return acc;
 }
 
-// Type-1 block of one triplet per wave (the triplets of one class
-// lambda = la + lb, listed in tlist).  chi[i][j] depends on (i, j) only
+// Type-1 block of one triplet per wave (the triplets tlist[0 .. ntl), in
+// class order lambda = la + lb).  chi[i][j] depends on (i, j) only
 // through the monomial p = i + j, so H[p] = sum_pairs sum_l F_pair[p][l]
 // Q[deg p][l] with F_pair[p][l] = sum_m S_lm(k^_pair) poly2sph[p][lm] is
 // formed once per monomial (same terms, same order as type1.c:266-295 per
