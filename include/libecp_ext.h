@@ -24,6 +24,27 @@ int libecp_screened_triplets(libECPHandle *h, int *keys, int cap);
    screening (libecp.c:256-312), the unit of the throughput metric. */
 long long libecp_nominal_triplets(libECPHandle *h);
 
+/* Phases of the last getIntegrals or calculateECPIntegrals call made by the
+   calling thread, in ms, up to n of: host tables (libECP_init; ~0 when they
+   come from the process cache), screening (triplet list), device context
+   (created, or taken from the cache), plan + upload, kernels, download,
+   matrix accumulation or callbacks.  Returns the number of phases (7). */
+int libecp_last_call_ms(double *ms, int n);
+
+/* Process-wide cache.  Handles with the default Cartesian ordering share
+   the geometry-independent tables of their key (largest ECP label, largest
+   basis l + n, grid order, tolerance, accuracy; at most 8 keys) and reuse
+   device contexts (streams, pinned staging, tables resident in HBM; at most
+   2 idle per key and device): libECP_free returns the context to the cache
+   instead of destroying it, and the next handle of the key takes it, so
+   repeated getIntegrals calls pay for neither.  Idle contexts keep their
+   device and pinned memory until libecp_cache_clear() (which frees them) or
+   the end of the process.  libecp_cache_enable(0) clears the cache and
+   disables it (every handle then builds and frees its own); returns the
+   previous setting. */
+void libecp_cache_clear(void);
+int libecp_cache_enable(int on);
+
 /* ---- density contraction on the device ----------------------------------
    The consumer sketched in getIntegrals.c:15-18 (a density matrix and a
    gradient in place of the AO matrix), evaluated where the blocks are: no

@@ -4,9 +4,11 @@
  */
 #define _GNU_SOURCE
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../../include/libecp.h"
 #include "../../include/getIntegrals.h"
@@ -21,6 +23,136 @@ const char *libecp_last_error(void) { return g_err; }
 
 void ecp_api_set_error(const char *msg) { set_last_error(msg); }
 
+/* Phases of the last getIntegrals / calculateECPIntegrals call on this
+   thread (ms): host tables, screening, device context, plan + upload,
+   kernels, download, matrix / callbacks. */
+enum { PH_TABLES = 0, PH_SCREEN, PH_CONTEXT, PH_UPLOAD, PH_KERNELS, PH_DOWNLOAD, PH_DELIVER, PH_N };
+static __thread double g_phase[PH_N];
+
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
+
+int libecp_last_call_ms(double *ms, int n) {
+  for (int i = 0; i < PH_N && i < n; i++) ms[i] = g_phase[i];
+  return PH_N;
+}
+
+/* ---- process-wide cache of static tables and device contexts -------------
+   A handle needs, besides its molecule, geometry-independent tables (grids,
+   Bessel table, angular and transformation tables: a few ms to build on the
+   host) and a device context (streams, events, pinned staging, the tables
+   resident in HBM).  Both depend only on (maxL, maxAlpha, grid order,
+   tolerance, accuracy) and the default Cartesian ordering, so they are kept
+   per process: the tables once per key, and up to kCacheIdle idle contexts
+   per key and device, which a handle takes at its first calculation and
+   returns when it is freed.  Handles with a custom ordering, and every
+   handle while the cache is disabled, own their tables and context. */
+enum { kCacheKeys = 8, kCacheIdle = 2 };
+typedef struct {
+  int used;
+  int maxL, maxAlpha, grid;
+  double tol, acc;
+  ecp_static st; /* read-only once built */
+  ecp_dev *idle[kCacheIdle];
+  int idle_dev[kCacheIdle];
+  int nidle;
+} cache_slot;
+
+static pthread_mutex_t g_cache_mu = PTHREAD_MUTEX_INITIALIZER;
+static cache_slot g_cache[kCacheKeys];
+static int g_cache_on = 1;
+
+/* the slot of a key, built on first use; NULL when the cache is off or full
+   (the caller then builds its own tables) */
+static cache_slot *cache_tables(int maxL, int maxAlpha, int grid, double tol, double acc, char *err,
+                                size_t errlen, int *failed) {
+  *failed = 0;
+  pthread_mutex_lock(&g_cache_mu);
+  cache_slot *hit = NULL, *fresh = NULL;
+  if (g_cache_on)
+    for (int i = 0; i < kCacheKeys; i++) {
+      cache_slot *c = &g_cache[i];
+      if (!c->used) {
+        if (!fresh) fresh = c;
+        continue;
+      }
+      if (c->maxL == maxL && c->maxAlpha == maxAlpha && c->grid == grid && c->tol == tol && c->acc == acc) {
+        hit = c;
+        break;
+      }
+    }
+  if (!hit && fresh) {
+    if (ecp_static_init(&fresh->st, maxL, maxAlpha, NULL, -1, grid, tol, acc, err, errlen)) {
+      *failed = 1;
+    } else {
+      fresh->used = 1;
+      fresh->maxL = maxL;
+      fresh->maxAlpha = maxAlpha;
+      fresh->grid = grid;
+      fresh->tol = tol;
+      fresh->acc = acc;
+      fresh->nidle = 0;
+      hit = fresh;
+    }
+  }
+  pthread_mutex_unlock(&g_cache_mu);
+  return hit;
+}
+
+static ecp_dev *cache_take(cache_slot *c, int device) {
+  ecp_dev *d = NULL;
+  pthread_mutex_lock(&g_cache_mu);
+  for (int i = c->nidle - 1; i >= 0; i--)
+    if (c->idle_dev[i] == device) {
+      d = c->idle[i];
+      c->idle[i] = c->idle[c->nidle - 1];
+      c->idle_dev[i] = c->idle_dev[c->nidle - 1];
+      c->nidle--;
+      break;
+    }
+  pthread_mutex_unlock(&g_cache_mu);
+  return d;
+}
+
+/* keeps an idle context for the next handle of the same key, or frees it */
+static void cache_give(cache_slot *c, ecp_dev *d, int device) {
+  if (!d) return;
+  pthread_mutex_lock(&g_cache_mu);
+  if (g_cache_on && c->nidle < kCacheIdle) {
+    c->idle[c->nidle] = d;
+    c->idle_dev[c->nidle] = device;
+    c->nidle++;
+    d = NULL;
+  }
+  pthread_mutex_unlock(&g_cache_mu);
+  ecp_dev_destroy(d);
+}
+
+void libecp_cache_clear(void) {
+  ecp_dev *drop[kCacheKeys * kCacheIdle];
+  int nd = 0;
+  pthread_mutex_lock(&g_cache_mu);
+  for (int i = 0; i < kCacheKeys; i++) {
+    for (int k = 0; k < g_cache[i].nidle; k++) drop[nd++] = g_cache[i].idle[k];
+    g_cache[i].nidle = 0;
+  }
+  pthread_mutex_unlock(&g_cache_mu);
+  /* the tables stay: handles alive may read them; they are a few MB */
+  for (int i = 0; i < nd; i++) ecp_dev_destroy(drop[i]);
+}
+
+int libecp_cache_enable(int on) {
+  pthread_mutex_lock(&g_cache_mu);
+  const int was = g_cache_on;
+  g_cache_on = on != 0;
+  pthread_mutex_unlock(&g_cache_mu);
+  if (!on) libecp_cache_clear();
+  return was;
+}
+
 struct _libECPHandle {
   int nat, n;
   /* private copies of the inputs (the reference borrows them, libecp.c:68-74) */
@@ -30,7 +162,10 @@ struct _libECPHandle {
   int *shellsBS, *lBS, *KBS;
   double *dBS, *aBS;
   ecp_mol mol;
-  ecp_static st;
+  const ecp_static *st; /* &st_own, or the cache's tables for this key */
+  ecp_static st_own;
+  cache_slot *slot;     /* non-NULL: tables and context come from the cache */
+  int dev_id;           /* HIP device of dev */
   ecp_dev *dev;
   ecp_job job;
   int job_ready;
@@ -48,8 +183,11 @@ static void *dupmem(const void *src, size_t bytes) {
 void libECP_free(libECPHandle *h) {
   if (!h) return;
   if (h->job_ready) ecp_job_free(&h->job);
-  ecp_dev_destroy(h->dev);
-  ecp_static_free(&h->st);
+  if (h->slot)
+    cache_give(h->slot, h->dev, h->dev_id);
+  else
+    ecp_dev_destroy(h->dev);
+  ecp_static_free(&h->st_own);
   free(h->geom);
   free(h->shellsECP);
   free(h->lECP);
@@ -117,12 +255,21 @@ libECPHandle *libECP_init(int nrAtoms, double *geometry, int *shellsECP, int *lE
                h->shellsBS, h->lBS, h->KBS, h->dBS, h->aBS};
   h->mol = m;
   const int maxL = d.maxLECP > 0 ? d.maxLECP : 1;
-  if (ecp_static_init(&h->st, maxL, d.maxLBS + n, shellOrdering, lmax, largeGridOrder, tolerance, accuracy, err,
-                      sizeof(err))) {
+  const double t0 = now_ms();
+  int failed = 0;
+  if (!shellOrdering)
+    h->slot = cache_tables(maxL, d.maxLBS + n, largeGridOrder, tolerance, accuracy, err, sizeof(err), &failed);
+  if (h->slot) {
+    h->st = &h->slot->st;
+  } else if (failed || ecp_static_init(&h->st_own, maxL, d.maxLBS + n, shellOrdering, lmax, largeGridOrder,
+                                       tolerance, accuracy, err, sizeof(err))) {
     set_last_error(err);
     libECP_free(h);
     return NULL;
+  } else {
+    h->st = &h->st_own;
   }
+  g_phase[PH_TABLES] = now_ms() - t0;
   return h;
 }
 
@@ -132,7 +279,9 @@ libECPHandle *libECP_init(int nrAtoms, double *geometry, int *shellsECP, int *lE
 static int ensure_device(libECPHandle *h) {
   char err[512] = "";
   if (h->dev) return 0;
-  h->dev = ecp_dev_create(&h->st, err, sizeof(err));
+  h->dev_id = ecp_dev_current();
+  if (h->slot) h->dev = cache_take(h->slot, h->dev_id);
+  if (!h->dev) h->dev = ecp_dev_create(h->st, err, sizeof(err));
   if (!h->dev) {
     set_last_error(err);
     return -1;
@@ -144,11 +293,13 @@ static int ensure_device(libECPHandle *h) {
    h->blocks / h->status; returns 0, or -1 on a device/host error. */
 static int run_handle(libECPHandle *h) {
   char err[512] = "";
+  for (int i = PH_SCREEN; i < PH_N; i++) g_phase[i] = 0.0;
+  double t = now_ms();
   if (h->job_ready) {
     ecp_job_free(&h->job);
     h->job_ready = 0;
   }
-  if (ecp_job_build(&h->mol, 1, h->n, &h->st, &h->job, err, sizeof(err))) {
+  if (ecp_job_build(&h->mol, 1, h->n, h->st, &h->job, err, sizeof(err))) {
     set_last_error(err);
     return -1;
   }
@@ -167,14 +318,36 @@ static int run_handle(libECPHandle *h) {
     set_last_error("out of host memory for the blocks");
     return -1;
   }
+  double t1 = now_ms();
+  g_phase[PH_SCREEN] = t1 - t;
   if (h->job.ntrip == 0) return 0;
   if (ensure_device(h)) return -1;
-  if (ecp_dev_run(h->dev, &h->job, h->blocks, h->status, err, sizeof(err))) {
+  t = now_ms();
+  g_phase[PH_CONTEXT] = t - t1;
+  /* prepare (host work lists, layout, upload) / execute (kernels, waits) /
+     fetch (download, waits); the wait after prepare only separates the
+     upload from the kernels in the phase timings */
+  if (ecp_dev_prepare(h->dev, &h->job, err, sizeof(err)) || ecp_dev_sync(h->dev)) {
+    set_last_error(err[0] ? err : "device error after the upload");
+    return -1;
+  }
+  t1 = now_ms();
+  g_phase[PH_UPLOAD] = t1 - t;
+  if (ecp_dev_execute(h->dev, 1, NULL, err, sizeof(err))) {
     set_last_error(err);
     return -1;
   }
+  t = now_ms();
+  g_phase[PH_KERNELS] = t - t1;
+  if (ecp_dev_fetch(h->dev, h->blocks, h->status, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  g_phase[PH_DOWNLOAD] = now_ms() - t;
   return 0;
 }
+
+static int deliver(libECPHandle *h, ECPCallback cb, void *args);
 
 int calculateECPIntegrals(libECPHandle *h, ECPCallback cb, void *args) {
   if (!h) {
@@ -182,6 +355,13 @@ int calculateECPIntegrals(libECPHandle *h, ECPCallback cb, void *args) {
     return -1;
   }
   if (run_handle(h)) return -1;
+  const double t0 = now_ms();
+  const int rc = deliver(h, cb, args);
+  g_phase[PH_DELIVER] = now_ms() - t0;
+  return rc;
+}
+
+static int deliver(libECPHandle *h, ECPCallback cb, void *args) {
   const ecp_job *J = &h->job;
   /* callbacks in the reference order (libecp.c:372); stop at the first
      failed integration with its documented code (libecp.h:23-26) */
@@ -215,6 +395,7 @@ int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *
     libECP_free(h);
     return 1;
   }
+  const double t0 = now_ms();
   if (h->job.ntrip > 0) {
     int *aoff = malloc(sizeof(int) * ((size_t)h->job.nsh + 1));
     if (!aoff) {
@@ -227,6 +408,7 @@ int getIntegrals(int nrAtoms, double *geometry, int *shellsECP, int *KECP, int *
     (void)ecp_job_scatter(&h->job, 0, aoff, h->blocks, h->status, rowdim, 0, I);
     free(aoff);
   }
+  g_phase[PH_DELIVER] = now_ms() - t0;
   libECP_free(h);
   return 0;
 }
@@ -248,7 +430,7 @@ int libecp_contract(libECPHandle *h, int rowdim, const double *P, double *out) {
     ecp_job_free(&h->job);
     h->job_ready = 0;
   }
-  if (ecp_job_build(&h->mol, 1, h->n, &h->st, &h->job, err, sizeof(err))) {
+  if (ecp_job_build(&h->mol, 1, h->n, h->st, &h->job, err, sizeof(err))) {
     set_last_error(err);
     return -1;
   }
@@ -288,7 +470,7 @@ int libecp_screened_triplets(libECPHandle *h, int *keys, int cap) {
     ecp_job_free(&h->job);
     h->job_ready = 0;
   }
-  if (ecp_job_build(&h->mol, 1, h->n, &h->st, &h->job, err, sizeof(err))) {
+  if (ecp_job_build(&h->mol, 1, h->n, h->st, &h->job, err, sizeof(err))) {
     set_last_error(err);
     return -1;
   }

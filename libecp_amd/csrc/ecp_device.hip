@@ -3628,6 +3628,14 @@ extern "C" void ecp_dev_mem_free(void *p) {
   if (p) (void)hipFree(p);
 }
 
+extern "C" int ecp_dev_sync(ecp_dev *d) {
+  int rc = 0;
+  if (d->stream && hipStreamSynchronize(d->stream) != hipSuccess) rc = -1;
+  for (int k = 0; k < kWorkers; k++)
+    if (d->workers[k] && hipStreamSynchronize(d->workers[k]) != hipSuccess) rc = -1;
+  return rc;
+}
+
 extern "C" int ecp_dev_current(void) {
   int dv = 0;
   return hipGetDevice(&dv) == hipSuccess ? dv : -1;

@@ -139,6 +139,10 @@ int ecp_dev_span_ms(ecp_dev *d, double *ms, int *samples, char *err, size_t errl
    graph (the next execute runs eagerly and captures again) */
 void ecp_dev_get_tuning(const ecp_dev *d, ecp_tuning *t);
 void ecp_dev_set_tuning(ecp_dev *d, const ecp_tuning *t);
+/* Waits for every stream of the context (after a failed bind / execute /
+   fetch, before its device memory is reused).  0, or -1 when a stream
+   reports an error. */
+int ecp_dev_sync(ecp_dev *d);
 /* device memory and the calling thread's current HIP device */
 int ecp_dev_mem_alloc(void **p, size_t bytes, char *err, size_t errlen);
 void ecp_dev_mem_free(void *p);

@@ -155,18 +155,29 @@ int ecp_static_init(ecp_static *st, int maxL, int maxAlpha, const int *ordering,
       return -1;
     }
     st->ijk = malloc(sizeof(int) * 3 * (size_t)ECP_DIM_C(lmaxOrdering));
-    memcpy(st->ijk, ordering, sizeof(int) * 3 * (size_t)ECP_DIM_C(lmaxOrdering));
+    if (st->ijk) memcpy(st->ijk, ordering, sizeof(int) * 3 * (size_t)ECP_DIM_C(lmaxOrdering));
     st->ijkDim = lmaxOrdering + 1;
-    st->ijkIndex = ecp_cart_index(lmaxOrdering, st->ijk);
+    st->ijkIndex = st->ijk ? ecp_cart_index(lmaxOrdering, st->ijk) : NULL;
+  }
+  if (!st->ijk || !st->ijkIndex) {
+    set_err(err, errlen, "out of host memory for the Cartesian ordering tables");
+    ecp_static_free(st);
+    return -1;
   }
   st->tmDim = tmDim;
   st->cart2sph = ecp_cart2sph(tmDim, st->ijk, st->fac);
-  st->poly2sph = ecp_poly2sph(st->cart2sph, tmDim, st->ijk, st->dfac);
+  st->poly2sph = st->cart2sph ? ecp_poly2sph(st->cart2sph, tmDim, st->ijk, st->dfac) : NULL;
   st->omL = maxL;
   st->omLambda = maxLambda;
   st->omAlpha = maxAlpha;
-  st->omega = ecp_omega(maxL, maxLambda, maxAlpha, st->ijk, st->ijkIndex, st->ijkDim, st->cart2sph,
-                        st->poly2sph, tmDim, st->dfac);
+  st->omega = st->poly2sph ? ecp_omega(maxL, maxLambda, maxAlpha, st->ijk, st->ijkIndex, st->ijkDim,
+                                       st->cart2sph, st->poly2sph, tmDim, st->dfac)
+                           : NULL;
+  if (!st->omega) {
+    set_err(err, errlen, "out of host memory for the angular tables");
+    ecp_static_free(st);
+    return -1;
+  }
   if (ecp_grid_ps93_kk(st->xs, st->ws) != ECP_SMALL_ORDER) {
     set_err(err, errlen, "small grid construction failed");
     ecp_static_free(st);
@@ -188,8 +199,10 @@ int ecp_static_init(ecp_static *st, int maxL, int maxAlpha, const int *ordering,
     return -1;
   }
   st->bessel = malloc(sizeof(double) * (ECP_BESSEL_N + 1) * ECP_BESSEL_STRIDE);
-  if (ecp_bessel_table(accuracy, ECP_BESSEL_N, 200, ECP_BESSEL_ROWS, st->bessel)) {
-    set_err(err, errlen, "Bessel function tabulation did not converge");
+  const int brc = st->bessel ? ecp_bessel_table(accuracy, ECP_BESSEL_N, 200, ECP_BESSEL_ROWS, st->bessel) : -1;
+  if (brc) {
+    set_err(err, errlen, brc < 0 ? "out of host memory for the Bessel table"
+                                 : "Bessel function tabulation did not converge");
     ecp_static_free(st);
     return -1;
   }

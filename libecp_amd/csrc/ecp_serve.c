@@ -48,6 +48,7 @@ typedef struct {
 typedef struct {
   char *base;
   size_t cap, used, peak;
+  size_t lost; /* bytes kept out of the pool after a failure */
   range *fr;
   int nfr, capfr;
 } pool;
@@ -70,13 +71,22 @@ static size_t pool_take(pool *p, size_t bytes) {
   return (size_t)-1;
 }
 
-static void pool_give(pool *p, size_t off, size_t bytes) {
+/* Returns the range to the free list: 0, or -1 when the free list cannot
+   grow (host memory); the range then stays out of the pool. */
+static int pool_give(pool *p, size_t off, size_t bytes) {
   bytes = (bytes + 255) & ~(size_t)255;
-  p->used -= bytes;
   int i = 0;
   while (i < p->nfr && p->fr[i].off < off) i++;
   const int joinl = i > 0 && p->fr[i - 1].off + p->fr[i - 1].size == off;
   const int joinr = i < p->nfr && off + bytes == p->fr[i].off;
+  if (!joinl && !joinr && p->nfr == p->capfr) {
+    const int nc = p->capfr ? 2 * p->capfr : 16;
+    range *nf = realloc(p->fr, sizeof(range) * (size_t)nc);
+    if (!nf) return -1;
+    p->fr = nf;
+    p->capfr = nc;
+  }
+  p->used -= bytes;
   if (joinl && joinr) {
     p->fr[i - 1].size += bytes + p->fr[i].size;
     memmove(p->fr + i, p->fr + i + 1, sizeof(range) * (size_t)(p->nfr - i - 1));
@@ -87,15 +97,12 @@ static void pool_give(pool *p, size_t off, size_t bytes) {
     p->fr[i].off = off;
     p->fr[i].size += bytes;
   } else {
-    if (p->nfr == p->capfr) {
-      p->capfr = p->capfr ? 2 * p->capfr : 16;
-      p->fr = realloc(p->fr, sizeof(range) * (size_t)p->capfr);
-    }
     memmove(p->fr + i + 1, p->fr + i, sizeof(range) * (size_t)(p->nfr - i));
     p->fr[i].off = off;
     p->fr[i].size = bytes;
     p->nfr++;
   }
+  return 0;
 }
 
 struct libecp_server;
@@ -108,6 +115,9 @@ typedef struct {
   size_t cap_blocks;
   unsigned char *status;
   size_t cap_status;
+  req **rs;     /* the requests of the current job */
+  ecp_mol *mols;
+  int cap_rs;
 } worker;
 
 struct libecp_server {
@@ -167,10 +177,9 @@ static void *worker_main(void *arg) {
   struct libecp_server *s = w->srv;
   char err[512];
   (void)ecp_dev_use(s->device);
-  int cap_rs = 64;
-  req **rs = malloc(sizeof(req *) * (size_t)cap_rs);
-  ecp_mol *mols = malloc(sizeof(ecp_mol) * (size_t)cap_rs);
   for (;;) {
+    req **rs = w->rs;
+    ecp_mol *mols = w->mols;
     /* take every queued request up to the batch cap */
     pthread_mutex_lock(&s->mu);
     while (!s->stop && !s->qhead) pthread_cond_wait(&s->work_cv, &s->mu);
@@ -182,16 +191,23 @@ static void *worker_main(void *arg) {
     long long nom = 0;
     while (s->qhead && n < s->max_batch_requests &&
            (n == 0 || nom + s->qhead->nominal <= s->max_batch_nominal)) {
+      if (n == w->cap_rs) {
+        /* grow the job's request arrays; without host memory the job keeps
+           the requests taken so far */
+        const int nc = 2 * w->cap_rs;
+        req **nr = realloc(w->rs, sizeof(req *) * (size_t)nc);
+        if (!nr) break;
+        w->rs = rs = nr;
+        ecp_mol *nm = realloc(w->mols, sizeof(ecp_mol) * (size_t)nc);
+        if (!nm) break;
+        w->mols = mols = nm;
+        w->cap_rs = nc;
+      }
       req *r = s->qhead;
       s->qhead = r->qnext;
       if (!s->qhead) s->qtail = NULL;
       s->queued--;
       r->state = RQ_RUNNING;
-      if (n == cap_rs) {
-        cap_rs *= 2;
-        rs = realloc(rs, sizeof(req *) * (size_t)cap_rs);
-        mols = realloc(mols, sizeof(ecp_mol) * (size_t)cap_rs);
-      }
       rs[n++] = r;
       nom += r->nominal;
     }
@@ -273,8 +289,12 @@ static void *worker_main(void *arg) {
           ecp_dev_fetch(w->dev, w->blocks, w->status, err, sizeof(err)))
         dev_rc = -1;
     }
+    /* after a device error, kernels of the job may still be queued on the
+       context's streams: wait for them before the job's memory is reused;
+       if that fails too, the range stays out of the pool */
+    const int reuse = !dev_rc || ecp_dev_sync(w->dev) == 0;
     pthread_mutex_lock(&s->mu);
-    if (bytes > 0) pool_give(&s->mem, off, bytes);
+    if (bytes > 0 && reuse && pool_give(&s->mem, off, bytes)) s->mem.lost += (bytes + 255) & ~(size_t)255;
     s->running--;
     pthread_cond_broadcast(&s->mem_cv);
     pthread_mutex_unlock(&s->mu);
@@ -306,8 +326,6 @@ static void *worker_main(void *arg) {
     pthread_cond_broadcast(&s->done_cv);
     pthread_mutex_unlock(&s->mu);
   }
-  free(rs);
-  free(mols);
   return NULL;
 }
 
@@ -316,6 +334,8 @@ static void server_release(struct libecp_server *s) {
     ecp_dev_destroy(s->wk[i].dev);
     free(s->wk[i].blocks);
     free(s->wk[i].status);
+    free(s->wk[i].rs);
+    free(s->wk[i].mols);
   }
   for (long long i = 0; i < s->ntab; i++)
     if (s->tab[i]) {
@@ -348,6 +368,10 @@ libecp_server *libecp_server_create(int slots, size_t hbm_budget, long long max_
     return NULL;
   }
   struct libecp_server *s = calloc(1, sizeof(*s));
+  if (!s) {
+    ecp_api_set_error("libecp_server_create: out of host memory");
+    return NULL;
+  }
   pthread_mutex_init(&s->mu, NULL);
   pthread_cond_init(&s->work_cv, NULL);
   pthread_cond_init(&s->mem_cv, NULL);
@@ -357,6 +381,11 @@ libecp_server *libecp_server_create(int slots, size_t hbm_budget, long long max_
   s->t0 = now_s();
   s->wk = calloc((size_t)slots, sizeof(worker));
   s->th = calloc((size_t)slots, sizeof(pthread_t));
+  if (!s->wk || !s->th) {
+    ecp_api_set_error("libecp_server_create: out of host memory");
+    server_release(s);
+    return NULL;
+  }
   /* static tables sized for every request this build accepts */
   if (ecp_static_init(&s->st, ECP_MAX_L, ECP_MAX_LBS, NULL, -1, largeGridOrder, tolerance, accuracy, err,
                       sizeof(err))) {
@@ -368,6 +397,15 @@ libecp_server *libecp_server_create(int slots, size_t hbm_budget, long long max_
   for (int i = 0; i < slots; i++) {
     s->wk[i].srv = s;
     s->wk[i].slot = i;
+    s->wk[i].cap_rs = 64;
+    s->wk[i].rs = malloc(sizeof(req *) * 64);
+    s->wk[i].mols = malloc(sizeof(ecp_mol) * 64);
+    if (!s->wk[i].rs || !s->wk[i].mols) {
+      s->nslots = i + 1;
+      ecp_api_set_error("libecp_server_create: out of host memory");
+      server_release(s);
+      return NULL;
+    }
     s->wk[i].dev = ecp_dev_create(&s->st, err, sizeof(err));
     if (!s->wk[i].dev) {
       s->nslots = i;
@@ -387,6 +425,11 @@ libecp_server *libecp_server_create(int slots, size_t hbm_budget, long long max_
   s->mem.cap = hbm_budget & ~(size_t)255;
   s->mem.capfr = 16;
   s->mem.fr = malloc(sizeof(range) * 16);
+  if (!s->mem.fr) {
+    ecp_api_set_error("libecp_server_create: out of host memory");
+    server_release(s);
+    return NULL;
+  }
   s->mem.fr[0].off = 0;
   s->mem.fr[0].size = s->mem.cap;
   s->mem.nfr = 1;
@@ -408,6 +451,10 @@ long long libecp_server_submit(libecp_server *s, const libecp_molecule *M) {
   char err[512] = "";
   if (!s || !M) return -1;
   req *r = calloc(1, sizeof(req));
+  if (!r) {
+    ecp_api_set_error("libecp_server_submit: out of host memory");
+    return -1;
+  }
   ecp_mol m0 = {M->nrAtoms, M->geometry, M->shellsECP, M->lECP, M->KECP, M->nECP, M->dECP, M->aECP,
                 M->shellsBS, M->lBS, M->KBS, M->dBS, M->aBS};
   ecp_mol_dims d;
@@ -420,12 +467,21 @@ long long libecp_server_submit(libecp_server *s, const libecp_molecule *M) {
   r->nsh = d.nsh;
   r->nominal = (long long)d.necp * d.nsh * (d.nsh + 1) / 2;
   pthread_mutex_lock(&s->mu);
+  if (s->ntab == s->captab) {
+    const long long nc = s->captab ? 2 * s->captab : 1024;
+    req **nt = realloc(s->tab, sizeof(req *) * (size_t)nc);
+    if (!nt) {
+      pthread_mutex_unlock(&s->mu);
+      ecp_mol_release(&r->mol);
+      free(r);
+      ecp_api_set_error("libecp_server_submit: out of host memory");
+      return -1;
+    }
+    s->tab = nt;
+    s->captab = nc;
+  }
   r->id = s->ntab;
   r->t_submit = now_s() - s->t0;
-  if (s->ntab == s->captab) {
-    s->captab = s->captab ? 2 * s->captab : 1024;
-    s->tab = realloc(s->tab, sizeof(req *) * (size_t)s->captab);
-  }
   s->tab[s->ntab++] = r;
   r->qnext = NULL;
   if (s->qtail)

@@ -38,6 +38,7 @@ double *ecp_factorials(int n, int step) {
 
 int *ecp_cart_order(int am) {
   int *t = calloc(3 * (size_t)ECP_DIM_C(am), sizeof(int));
+  if (!t) return NULL;
   for (int l = 0; l <= am; l++) {
     int c = 0;
     for (int i = 0; i <= l; i++)
@@ -54,6 +55,7 @@ int *ecp_cart_order(int am) {
 int *ecp_cart_index(int am, const int *ijk) {
   const int dim = am + 1;
   int *idx = calloc((size_t)dim * dim * dim, sizeof(int));
+  if (!idx) return NULL;
   for (int l = 0; l <= am; l++)
     for (int c = 0; c < ECP_DIM_IJK(l); c++) {
       int p = 3 * ECP_IDX_C(l, c);
@@ -76,6 +78,7 @@ static double sgn_pow(int e) { return pow(-1.0, e); }
 
 double *ecp_cart2sph(int lmax, const int *xyz, const double *fac) {
   double *TM = calloc((size_t)ecp_tm_offset(lmax + 1) + 1, sizeof(double));
+  if (!TM) return NULL;
   double *T = TM;
   for (int l = 0; l <= lmax; l++)
     for (int m = -l; m <= l; m++) {
@@ -113,6 +116,7 @@ double *ecp_cart2sph(int lmax, const int *xyz, const double *fac) {
 double *ecp_poly2sph(const double *M, int lmax, const int *xyz, const double *fac2) {
   const int ldim = ECP_DIM_L(lmax);
   double *TM = calloc((size_t)ECP_DIM_C(lmax) * ldim, sizeof(double));
+  if (!TM) return NULL;
   for (int l1 = 0; l1 <= lmax; l1++)
     for (int c1 = 0; c1 < ECP_DIM_IJK(l1); c1++) {
       const int *e1 = xyz + 3 * ECP_IDX_C(l1, c1);
@@ -292,6 +296,11 @@ int ecp_bessel_table(double accuracy, int N, int cutoff, int rows, double *Kdev)
   double *F = calloc((size_t)cutoff + 2, sizeof(double));
   double *G = calloc((size_t)cutoff + lMax + 3, sizeof(double));
   int rc = 0;
+  if (!F || !G) {
+    free(F);
+    free(G);
+    return -1;
+  }
   memset(Kdev, 0, sizeof(double) * (size_t)(N + 1) * ECP_BESSEL_STRIDE);
   Kdev[0] = 1.0;
   for (int i = 1; i <= N && !rc; i++) {
@@ -336,6 +345,7 @@ double *ecp_omega(int maxL, int maxLambda, int maxAlpha, const int *ijk, const i
   const int d1 = ECP_DIM_L(maxLambda), d2 = ECP_DIM_C(maxAlpha);
   const int tmd = ECP_DIM_L(tmDim);
   double *O = calloc((size_t)ECP_DIM_L(maxL) * d1 * d2 + 1, sizeof(double));
+  if (!O) return NULL;
   for (int lambda = 0; lambda <= maxLambda; lambda++)
     for (int l = 0; l < maxL; l++) {
       const int par = (lambda + l) % 2, lo = lambda - l;

@@ -28,7 +28,30 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.libECP_init.restype = C.c_void_p
         _lib.libecp_last_error.restype = C.c_char_p
+        _lib.libecp_screened_triplets.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _lib.libecp_last_call_ms.argtypes = [C.c_void_p, C.c_int]
+        _lib.libecp_cache_enable.argtypes = [C.c_int]
     return _lib
+
+
+PHASES = ("tables", "screening", "context", "upload", "kernels", "download", "deliver")
+
+
+def last_call_ms() -> dict:
+    """Phases (ms) of this thread's last getIntegrals / calculateECPIntegrals."""
+    out = np.zeros(len(PHASES))
+    lib().libecp_last_call_ms(out.ctypes.data, len(PHASES))
+    return dict(zip(PHASES, out.tolist()))
+
+
+def cache_enable(on: bool) -> bool:
+    """Enables or disables (and clears) the process cache of tables and
+    device contexts; returns the previous setting."""
+    return bool(lib().libecp_cache_enable(1 if on else 0))
+
+
+def cache_clear() -> None:
+    lib().libecp_cache_clear()
 
 
 def last_error() -> str:
@@ -162,6 +185,17 @@ class Handle:
         if rc < 0:
             raise RuntimeError("libecp_contract failed: " + last_error())
         return rc, (float(out[0]) if self.n == 0 else out.reshape(-1, 3))
+
+    def screened_triplets(self):
+        """(C, A, sa, B, sb) of every triplet calculateECPIntegrals evaluates,
+        in callback order (host-side screening, no device work)."""
+        L = lib()
+        n = L.libecp_screened_triplets(self.h, None, 0)
+        if n < 0:
+            raise RuntimeError(last_error())
+        keys = np.zeros((max(n, 1), 5), dtype=np.int32)
+        L.libecp_screened_triplets(self.h, keys.ctypes.data, n)
+        return keys[:n]
 
     def close(self):
         if self.h:

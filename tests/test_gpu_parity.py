@@ -197,7 +197,12 @@ def test_high_l_against_oracle_with_rounding_envelope():
                            for x1, c1 in zip(aa, da) for x2, c2 in zip(ab, db) for _, n, d, g in local[Cat])
                        for ea in cart_exps(la) for eb in cart_exps(lb)])
         assert np.all(np.abs(a - ex) <= 2 * np.abs(b - ex) + ATOL + RTOL * np.abs(ex)), key
-    assert outside <= 10
+    # the device rounds as the reference does (no contraction); one type-1
+    # block of two g shells on one atom (A = B = 19, C = 5) stays outside
+    # the envelope: its values, ~1e-5 in a block of 1e-4, come from
+    # cancellation in the C-centred expansion, and it is checked against
+    # the exact integral above
+    assert outside <= 1
 
 
 def test_failure_code_and_delivered_prefix():

@@ -1,14 +1,16 @@
-"""The reference's own rounding floor on the medium cluster.
+"""The reference's own rounding floor on the medium cluster, and the device
+inside the strict bar.
 
 The oracle is bit-identical to the reference.  Compiled with FMA
 contraction (oracle/liboracle_fma.so: the same source, only the rounding
 of a*b+c changes) it moves the medium cluster's getIntegrals matrix by up
 to 5.2e-10 absolute (10 elements above 1e-10): ulp-level changes move the
 stopping level of an adaptive quadrature, and the element moves by that
-level's error.  The device (its own exp, FMA contraction) differs from the
-reference by the same mechanism, so the bar for it is this floor: no
-further from the reference than the reference's own FMA build, and every
-element within 1e-10 absolute + 1e-8 relative.
+level's error.  The device library is compiled without contraction
+(-ffp-contract=off), so its integrands and level sums round as the
+reference's do; what remains is its own exp/pow/trigonometric rounding.
+Bar for the device: every element within 1e-10 absolute of the reference
+and no further from it than the FMA build.
 """
 import os
 
@@ -40,10 +42,10 @@ def test_fma_build_of_the_reference_algorithm_moves_the_matrix():
 
 
 @pytest.mark.gpu
-def test_device_within_the_reference_rounding_floor():
+def test_device_within_1e10_of_the_reference():
     m, ref, fma = _matrices()
     rc, I = native.get_integrals(m)
     assert rc == 0
     d = np.abs(I - ref)
     assert d.max() <= np.abs(fma - ref).max(), (d.max(), np.abs(fma - ref).max())
-    assert np.all(d <= 1e-10 + 1e-8 * np.abs(ref))
+    assert d.max() <= 1e-10, (float(d.max()), int((d > 1e-10).sum()))
