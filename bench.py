@@ -9,12 +9,23 @@ before the timed region) and the device kernel sequence replayed from a HIP
 graph.  `value` = nominal triplets / wall time per step (the reference's
 triplet loop count before screening, libecp.c:256-312).
 
-Also reported: the plain per-call drop-in (getIntegrals, host arrays in and
-out, every call building its handle as the reference does), the uneven
-batch of 256 molecules in one resident job, the dominant kernel against the
-FP64 peak, and the reference C library on the host's cores.
+Legs beside it (each also reports evaluated, i.e. post-screening,
+triplets/s):
+  plain        getIntegrals per call on the medium cluster (host arrays in
+               and out), with the process cache of tables and device
+               contexts warm and cold, phase by phase
+  small        the 4-atom molecule, getIntegrals per call, against the
+               reference per call
+  high_l       the 20-atom g-shell configuration in one resident job, and
+               its distance from the oracle element by element
+  batched      256 molecules in one resident job (autotuned launch
+               parameters), fresh-job end to end, and the same molecules
+               through getIntegrals one by one
+  serving      a Poisson request stream against the request server
+  contraction  energy and gradient on the device
+  cpu          the reference C library on the host's cores
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+    python bench.py [--steps K] [--warmup W] [--no-<leg> ...] [--only <leg>]
 """
 from __future__ import annotations
 
@@ -32,13 +43,15 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from libecp_amd import native  # noqa: E402
-from libecp_amd.workloads import medium_cluster, serving_requests, uneven_batch  # noqa: E402
+from libecp_amd.workloads import high_l, medium_cluster, plain_small, serving_requests, uneven_batch  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_medium_cluster.md")
+
 
 
 def measured_traffic():
@@ -51,6 +64,32 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def host_info():
+    ncpu = os.cpu_count()
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else ncpu
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": ncpu, "usable_cpus": aff, "cpu_model": model}
+
+
+def usable_cpus():
+    return host_info()["usable_cpus"]
+
+
+def screened_count(m):
+    h = native.Handle(m)
+    try:
+        return len(h.screened_triplets())
+    finally:
+        h.close()
 
 
 def ecp_subset(m, keep):
@@ -82,43 +121,73 @@ def ecp_subset(m, keep):
     return mm
 
 
-def _ref_worker(args):
-    m, keep = args
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+def _cpu_worker(args):
+    """One CPU getIntegrals (the reference, -O2 or -O0, or the oracle) on a
+    molecule, optionally restricted to some ECP centres; (matrix, seconds)."""
     import ecp_native as E
-    mm = ecp_subset(m, keep)
+    m, keep, which = args
+    mm = m if keep is None else ecp_subset(m, keep)
     t = time.perf_counter()
-    rc, I = E.ref_getIntegrals(mm)
+    if which == "oracle":
+        rc, I = E.oracle_getIntegrals(mm, options=1)
+    else:
+        rc, I = E.ref_getIntegrals(mm, lib=E.load(E.REF_O0_SO if which == "ref_o0" else E.REF_SO))
     return I, time.perf_counter() - t
 
 
-def cpu_baseline(m, nominal):
+def cpu_pool(jobs, workers):
+    """_cpu_worker over jobs in forked processes; a worker that dies raises
+    (BrokenProcessPool) instead of leaving the pool waiting."""
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("fork")) as ex:
+        t = time.perf_counter()
+        res = list(ex.map(_cpu_worker, jobs))
+        return res, time.perf_counter() - t
+
+
+def split_by_centre(m, which, workers):
+    """getIntegrals of m split by ECP centre over worker processes (the
+    contributions add per centre); (summed matrix, wall seconds)."""
+    centres = [a for a, n in enumerate(m.shellsECP) if n > 0]
+    w = max(1, min(workers, len(centres)))
+    groups = [set(centres[i::w]) for i in range(w)]
+    res, wall = cpu_pool([(m, g, which) for g in groups], w)
+    return sum(r[0] for r in res), wall, w
+
+
+def cpu_baseline(m, nominal, evaluated):
     """The reference library (compiled from its sources into oracle/_ref) on
-    the host: the full workload single-threaded, then split by ECP centre
-    over worker processes."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    the host: the medium cluster single-threaded (-O2 and as shipped, -O0),
+    split by ECP centre over processes (latency of one molecule), and one
+    copy per usable core at once (throughput with every core busy)."""
     import ecp_native as E
     if not E.have_ref():
         return None, None
-    t = time.perf_counter()
-    rc, I1 = E.ref_getIntegrals(m)
-    t1 = time.perf_counter() - t
-    centres = [a for a, n in enumerate(m.shellsECP) if n > 0]
-    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    workers = max(1, min(ncpu, 16, len(centres)))
-    groups = [centres[i::workers] for i in range(workers)]
-    t = time.perf_counter()
-    with mp.get_context("fork").Pool(workers) as pool:
-        parts = pool.map(_ref_worker, [(m, set(g)) for g in groups])
-    tn = time.perf_counter() - t
-    In = sum(p[0] for p in parts)
-    split_err = float(np.abs(In - I1).max())
-    single = dict(value=nominal / t1, unit="shell-triplets/s", cores=1, kind="reference",
-                  sample=f"medium_cluster(seed=0) in full, one getIntegrals call, {t1:.2f} s")
-    multi = dict(value=nominal / tn, unit="shell-triplets/s", cores=workers, kind="reference",
-                 sample=f"medium_cluster(seed=0) in full, split by ECP centre over {workers} processes "
-                        f"({tn:.2f} s; max |sum - single| = {split_err:.1e})")
-    return (single, multi), I1
+    info = host_info()
+    ncpu = info["usable_cpus"]
+    (I1, t1), = [_cpu_worker((m, None, "ref"))]
+    out = {}
+    out["single_core"] = dict(value=nominal / t1, evaluated_per_s=evaluated / t1, unit="shell-triplets/s", cores=1,
+                              kind="reference", seconds=t1,
+                              sample="medium_cluster(seed=0) in full, one getIntegrals call, -O2")
+    if os.path.exists(E.REF_O0_SO):
+        (_, t0), = [_cpu_worker((m, None, "ref_o0"))]
+        out["single_core_as_shipped_O0"] = dict(value=nominal / t0, unit="shell-triplets/s", cores=1,
+                                                kind="reference", seconds=t0,
+                                                sample="medium_cluster(seed=0) in full, the as-shipped build (no -O)")
+    In, tn, w = split_by_centre(m, "ref", min(ncpu, 16))
+    out["split_by_centre"] = dict(value=nominal / tn, unit="shell-triplets/s", cores=w, kind="reference",
+                                  seconds=tn, max_abs_sum_vs_single=float(np.abs(In - I1).max()),
+                                  sample=f"medium_cluster(seed=0), its 10 ECP centres split over {w} processes")
+    ncopy = max(1, min(ncpu, 16))
+    res, tc = cpu_pool([(m, None, "ref")] * ncopy, ncopy)
+    mx = max(float(np.abs(r[0] - I1).max()) for r in res)
+    main = dict(value=ncopy * nominal / tc, evaluated_per_s=ncopy * evaluated / tc, unit="shell-triplets/s",
+                cores=ncopy, kind="reference", seconds=tc, host=info,
+                sample=f"{ncopy} copies of medium_cluster(seed=0) at once, one per process and core "
+                       f"({ncopy} x 128,800 nominal triplets in {tc:.2f} s; every copy equal to the single run "
+                       f"within {mx:.1e})")
+    return (main, out), I1
 
 
 def _pct(v, q):
@@ -134,6 +203,7 @@ def serve_bench(n_req, slots, budget_gib, max_batch, load=0.7, seed=0):
     reqs = serving_requests(seed=seed, n=n_req)
     mols = [m for m, _ in reqs]
     nominal = np.array([m.nominal_triplets() for m in mols], dtype=np.float64)
+    evaluated = np.array([screened_count(m) for m in mols], dtype=np.float64)
     S = native.Server(slots=slots, hbm_budget=int(budget_gib * (1 << 30)), max_batch_triplets=max_batch)
     try:
         # warm the workers (first use of each context)
@@ -196,6 +266,8 @@ def serve_bench(n_req, slots, budget_gib, max_batch, load=0.7, seed=0):
         diff = max(diff, float(np.abs(I - I_alone).max()))
     return {
         "value": nominal.sum() / (last - first), "unit": "shell-triplets/s",
+        "evaluated_per_s": evaluated.sum() / (last - first),
+        "nominal_triplets": int(nominal.sum()), "evaluated_triplets": int(evaluated.sum()),
         "p50_latency_ms": _pct(lat, 50), "p99_latency_ms": _pct(lat, 99), "mean_latency_ms": float(np.mean(lat)),
         "p50_admission_wait_ms": _pct(wait, 50), "p99_admission_wait_ms": _pct(wait, 99),
         "requests": len(mols), "failed": failed,
@@ -284,38 +356,8 @@ def contract_bench(m, reps=10):
     return res
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-contract", action="store_true")
-    ap.add_argument("--autotune", action="store_true",
-                    help="run the launch-parameter autotuner on each job before timing (default: built-in choices)")
-    ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-budget-gib", type=float, default=32.0)
-    ap.add_argument("--serve-max-batch", type=int, default=4000000)
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run one device copy of 256 MiB (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build evaluates on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding by ECP centre across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
-        sys.exit(2)
 
-    if args.calibrate_copy:
-        import torch
-        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-
+def leg_main(args):
     m = medium_cluster(0)
     b = native.Batch([m])
     nominal, evaluated = b.triplets()
@@ -327,18 +369,14 @@ def main():
     dev_ms = b.execute(args.steps)
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
-    value = nominal / (wall / args.steps)
-    # the type-1 quadrature's duration inside the timed region's graph
-    # replays (mean over its last replays, one per captured graph copy)
     quad_ms = b.quadrature_ms()
     quad_n = getattr(b, "quadrature_samples", 0)
     b.fetch()
     rc, I_gpu = b.matrix(0)
     b.close()
-
     out = {
         "metric": "ECP shell-triplets/s FP64 (batched vs plain, 1 GPU); p50/p99 req latency",
-        "value": value,
+        "value": nominal / (wall / args.steps),
         "unit": "shell-triplets/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -358,97 +396,300 @@ def main():
             "evaluated_triplets": evaluated,
             "largeGridOrder": 1024, "tolerance": 1e-12, "accuracy": 1e-14,
         },
+        "evaluated_per_s": evaluated / (wall / args.steps),
         "device_ms_per_step": dev_ms,
         "autotune": tune_report,
-        "kernels": {k: {"launches": v[0], "ms": v[1], "algorithmic_gflop": v[2] / 1e9} for k, v in prof.items()},
-        "stage_ms_first_step": dict(zip(["tables", "quadratures", "assembly"], stages[:3])),
         "status": rc,
     }
-
-    # Roofline of the dominant kernel with an algorithmic model: its FLOPs per
-    # evaluation under the reference algorithm's per-unit counts (type 1:
-    # G (31 lambda + 74 + 4 n_Q) + 150 n_Q per primitive pair; type 2:
-    # 4 G + 150 per T entry) over its summed launch durations, both measured
-    # above with HIP events on the launches' own streams.  The type-1
-    # small-grid integration is two kernels (k_type1 runs the first PS93
-    # levels lane-serially, k_t1tail finishes the open pairs wave-
-    # cooperatively); their durations are summed against its FLOPs.
+    # every kernel family with its algorithmic FLOPs (the reference's eager
+    # algorithm: every window point tabulated, every fallback on all its
+    # large-grid points) against its device time in one eager evaluation
+    fam = {}
+    for k, (launches, ms, flops) in prof.items():
+        e = {"launches": launches, "ms": ms, "algorithmic_gflop": flops / 1e9}
+        if flops > 0 and ms > 0:
+            e["tflops"] = flops / (ms * 1e-3) / 1e12
+            e["frac_of_fp64_peak"] = e["tflops"] / FP64_PEAK_TFLOPS
+        fam[k] = e
+    out["kernels"] = fam
+    out["kernels_note"] = ("eager evaluation, HIP events per launch on its stream; families overlap on the "
+                           "device, so their times do not add up to the step")
+    out["stage_ms_first_step"] = dict(zip(["tables", "quadratures", "assembly"], stages[:3]))
+    # Roofline of the type-1 small-grid quadrature, the largest algorithmic
+    # cost: G (31 lambda + 74 + 4 n_Q) + 150 n_Q per primitive pair, over the
+    # pair of launches that evaluates it (k_type1 runs the first PS93 levels
+    # lane-serially, k_t1tail finishes the open pairs wave-cooperatively),
+    # timed by HIP events on their stream inside the timed graph replays.
     t1_launch, t1_ms, t1_flops = prof["type1"]
-    tail_launch, tail_ms = prof.get("type1_tail", (0, 0.0, 0.0))[:2]
-    fams = {"type1": (t1_launch, t1_ms + tail_ms, t1_flops), "type2": prof["type2"]}
-    fam = max((k for k in fams if fams[k][0] > 0), key=lambda k: fams[k][1])
-    launches, kms, kflops = fams[fam]
-    quad_ok = fam == "type1" and quad_ms is not None
-    if quad_ok:
-        kms = quad_ms
-    achieved = kflops / (kms * 1e-3) / 1e12
-    out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                       "frac": achieved / FP64_PEAK_TFLOPS,
-                       "traffic": measured_traffic() if fam == "type1" else None,
-                       "kernel": "type-1 small grid (k_type1*, k_t1tail*)" if fam == "type1" else "k_type2",
-                       "launches_per_step": launches,
-                       "avg_launch_ms": kms / launches, "algorithmic_flop_per_launch": kflops / launches,
-                       "note": "FP64 vector work (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s); "
-                               "algorithmic FLOPs count every window point as the reference tabulates them, "
-                               "the kernels evaluate points lazily up to convergence"}
-    if fam == "type1":
-        out["roofline"]["duration_source"] = (
-            f"HIP events on the quadrature's stream inside the timed region's graph replays (mean of its last "
-            f"{quad_n})" if quad_ok else "one eager evaluation, HIP events per launch")
-        out["roofline"]["eager_ms_k_type1"] = t1_ms / max(t1_launch, 1)
-        out["roofline"]["eager_ms_k_t1tail"] = tail_ms / max(tail_launch, 1)
-    step_flops = prof["type1"][2] + prof["type2"][2]
-    out["algorithmic_tflops_per_step"] = step_flops / (ms_per_step * 1e-3) / 1e12
+    tail_ms = prof.get("type1_tail", (0, 0.0, 0.0))[1]
+    kms = quad_ms if quad_ms is not None else t1_ms + tail_ms
+    achieved = t1_flops / (kms * 1e-3) / 1e12
+    out["roofline"] = {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
+                       "kernel": "type-1 small grid (k_type1*, k_t1tail*)",
+                       "launches_per_step": 1, "avg_launch_ms": kms,
+                       "algorithmic_flop_per_launch": t1_flops,
+                       "duration_source": (f"HIP events on the quadrature's stream inside the timed region's graph "
+                                           f"replays (mean of its last {quad_n})" if quad_ms is not None
+                                           else "one eager evaluation, HIP events per launch"),
+                       "eager_ms_k_type1": t1_ms, "eager_ms_k_t1tail": tail_ms,
+                       "note": "FP64 vector (VALU) work: exponentials, Bessel recurrences, FMA chains; MI355X FP64 "
+                               "vector and matrix peaks are both 78.6 TFLOP/s.  Algorithmic FLOPs count every "
+                               "window point as the reference tabulates them; the kernels evaluate points lazily "
+                               "up to convergence"}
+    total = sum(v[2] for v in prof.values())
+    out["algorithmic_tflops_per_step"] = total / (ms_per_step * 1e-3) / 1e12
+    out["algorithmic_frac_per_step"] = out["algorithmic_tflops_per_step"] / FP64_PEAK_TFLOPS
+    return out, m, I_gpu
 
+
+def _median_call(m, reps, cold):
+    """Median wall time of getIntegrals on m and the phases of that call."""
+    times, phases = [], []
+    for _ in range(reps):
+        if cold:
+            native.cache_clear()
+            was = native.cache_enable(False)
+        t = time.perf_counter()
+        rc, I = native.get_integrals(m)
+        times.append(time.perf_counter() - t)
+        phases.append(native.last_call_ms())
+        if cold:
+            native.cache_enable(was)
+        if rc != 0:
+            raise RuntimeError(f"getIntegrals returned {rc}")
+    k = int(np.argsort(times)[len(times) // 2])
+    return times[k], phases[k], I
+
+
+def leg_plain(m, I_gpu, nominal, evaluated):
+    """getIntegrals per call on the medium cluster: host arrays in, handle,
+    screening, upload, kernels, download and the host matrix every call.
+    Warm: the process cache of tables and device contexts holds this key
+    (the state of every call after a process's first); cold: the cache off,
+    tables and device context built and freed in the call."""
+    native.get_integrals(m)  # fills the cache
+    tw, pw, Iw = _median_call(m, 7, cold=False)
+    tc, pc, _ = _median_call(m, 3, cold=True)
+    return {"value": nominal / tw, "evaluated_per_s": evaluated / tw, "unit": "shell-triplets/s",
+            "ms_per_call": 1e3 * tw, "phases_ms": pw,
+            "cold": {"value": nominal / tc, "ms_per_call": 1e3 * tc, "phases_ms": pc},
+            "what": "getIntegrals per call: host arrays in, screening, upload, eager kernels, download, "
+                    "upper-triangle matrix out; warm = tables and device context from the process cache, "
+                    "cold = built and freed in the call",
+            "max_abs_vs_resident": float(np.abs(Iw - I_gpu).max())}
+
+
+def leg_small(args, ctx):
+    """The 4-atom molecule (78 nominal triplets) per getIntegrals call; the
+    matrices of seeds 0..9 are kept for the CPU leg's parity check."""
+    m = plain_small(0)
+    native.get_integrals(m)
+    t, ph, I = _median_call(m, 21, cold=False)
+    nominal = m.nominal_triplets()
+    evaluated = screened_count(m)
+    ctx["small"] = [(plain_small(s), native.get_integrals(plain_small(s))[1]) for s in range(10)]
+    return {"value": nominal / t, "evaluated_per_s": evaluated / t, "unit": "shell-triplets/s",
+            "ms_per_call": 1e3 * t, "phases_ms": ph, "nominal_triplets": nominal, "evaluated_triplets": evaluated,
+            "workload": "plain_small(seed=0): 4 atoms in a 10-bohr box, 1 ECP centre L=3, basis s,p,d (K=3), "
+                        "40 AOs"}
+
+
+def leg_high_l(args, ctx):
+    """The 20-atom configuration with g shells and L = 5 projectors (Bessel
+    orders up to 13, lambda up to 8) in one resident job."""
+    m = high_l(0)
+    b = native.Batch([m])
+    nominal, evaluated = b.triplets()
+    b.execute(2)
+    steps = max(3, args.steps // 4)
+    t = time.perf_counter()
+    b.execute(steps)
+    w = (time.perf_counter() - t) / steps
+    b.fetch()
+    rc, I = b.matrix(0)
+    b.close()
+    ctx["high_l"] = (m, I)
+    return {"value": nominal / w, "evaluated_per_s": evaluated / w, "unit": "shell-triplets/s",
+            "ms_per_step": 1e3 * w, "nominal_triplets": nominal, "evaluated_triplets": evaluated, "status": rc,
+            "workload": "high_l(seed=0): 20 atoms in a 15-bohr box, 8 ECP centres L=5 (s..g projectors), basis "
+                        "s..g on every atom (K=4, exponents 0.05..100), 700 AOs, one resident job"}
+
+
+def leg_batched(args, ctx):
+    mols = uneven_batch(0)
+    t = time.perf_counter()
+    bb = native.Batch(mols)
+    t_create = time.perf_counter() - t
+    t = time.perf_counter()
+    bb.execute(1)
+    t_first = time.perf_counter() - t
+    nom_b, ev_b = bb.triplets()
+    tune_b = bb.autotune(2) if not args.no_autotune else "defaults"
+    params = {k: bb.get_param(k) for k in ("fuse", "asm2_cache", "t1_head_coop", "t1_head_serial", "t1_coop_max",
+                                           "t1_tail_grid", "t2_head", "t2_tail_grid", "fb2_split")}
+    bb.execute(2)
+    st_b = bb.stage_ms()
+    t = time.perf_counter()
+    ms_b = bb.execute(5)
+    wb = (time.perf_counter() - t) / 5
+    bb.close()
+    out = {"value": nom_b / wb, "evaluated_per_s": ev_b / wb, "unit": "shell-triplets/s", "ms_per_step": 1e3 * wb,
+           "device_ms_per_step": ms_b, "nominal_triplets": nom_b, "evaluated_triplets": ev_b,
+           "fresh_job": {"host_setup_s": t_create, "first_evaluation_s": t_first,
+                         "end_to_end_value": nom_b / (t_create + t_first),
+                         "what": "host screening + job upload, then the first (eager) evaluation"},
+           "autotune": tune_b, "launch_parameters": params,
+           "stage_ms_first_step": dict(zip(["tables", "quadratures", "assembly"], st_b[:3])),
+           "workload": "uneven_batch(seed=0): 256 molecules, 3..40 atoms, 20% ECP atoms (L=4), basis l <= 3, "
+                       "one resident job replayed from a HIP graph"}
+    # the same molecules through the plain drop-in, one getIntegrals each
+    for m in mols[:4]:
+        native.get_integrals(m)
+    t = time.perf_counter()
+    for m in mols:
+        native.get_integrals(m)
+    tp = time.perf_counter() - t
+    out["plain_same_molecules"] = {"value": nom_b / tp, "seconds": tp,
+                                   "what": "getIntegrals per molecule, 256 calls, process cache warm"}
+    out["vs_plain_same_molecules"] = tp / wb
+    ctx["batched"] = (nom_b / wb, mols)
+    return out
+
+
+def cpu_checks(ctx, out):
+    """The CPU side of the legs that ran on the GPU: the reference per
+    call on the 4-atom molecule and its parity on seeds 0..9; the oracle on
+    the high-l configuration (the reference itself aborts there: a double
+    free after a large-grid integration fails on its stale buffers), split by
+    ECP centre over processes, element by element against the device; the
+    reference on every 8th molecule of the uneven batch, one per process."""
+    import ecp_native as E
+    ncpu = max(1, min(usable_cpus(), 16))
+    if "small" in ctx and E.have_ref():
+        m = ctx["small"][0][0]
+        reps = 10
+        tr = time.perf_counter()
+        for _ in range(reps):
+            E.ref_getIntegrals(m)
+        tr = (time.perf_counter() - tr) / reps
+        o = out["small"]
+        o["cpu_reference"] = {"ms_per_call": 1e3 * tr, "cores": 1, "kind": "reference",
+                              "value": m.nominal_triplets() / tr}
+        o["vs_cpu_reference"] = tr / (o["ms_per_call"] * 1e-3)
+        worst, above = 0.0, 0
+        for ms, I in ctx["small"]:
+            d = np.abs(I - E.ref_getIntegrals(ms)[1])
+            worst, above = max(worst, float(d.max())), above + int((d > 1e-10).sum())
+        o["max_abs_vs_reference_seeds_0_9"] = worst
+        o["elements_above_1e-10_abs"] = above
+    if "high_l" in ctx:
+        m, I = ctx["high_l"]
+        R, wall, w_ = split_by_centre(m, "oracle", ncpu)
+        d = np.abs(I - R)
+        big = np.abs(R) > 1e-6
+        rel = d[big] / np.abs(R[big])
+        out["high_l"].update({
+            "oracle_cpu": {"value": m.nominal_triplets() / wall, "unit": "shell-triplets/s", "cores": w_,
+                           "kind": "port", "seconds": wall, "sample": "high_l(seed=0) in full, split by ECP centre"},
+            "max_abs_vs_oracle": float(d.max()),
+            "oracle_at_max_abs": float(R.flat[int(np.argmax(d))]),
+            "elements_above_1e-10_abs": int((d > 1e-10).sum()),
+            "elements_outside_1e-10_abs_1e-8_rel": int((d > 1e-10 + 1e-8 * np.abs(R)).sum()),
+            "max_rel_where_abs_ref_above_1e-6": float(rel.max()) if rel.size else 0.0,
+            "elements": int(d.size)})
+    if "batched" in ctx:
+        # the oracle, not the reference: on some of these molecules a
+        # large-grid integration fails on the reference's stale buffers and
+        # its error path frees an array twice (the process aborts)
+        value, mols = ctx["batched"]
+        sample = mols[::8]
+        res, tc = cpu_pool([(m, None, "oracle") for m in sample], ncpu)
+        nom_s = sum(m.nominal_triplets() for m in sample)
+        out["batched"]["cpu_oracle"] = {
+            "value": nom_s / tc, "unit": "shell-triplets/s", "cores": ncpu, "kind": "port", "seconds": tc,
+            "sample": f"every 8th molecule ({len(sample)} of 256, {nom_s} nominal triplets), one per process"}
+        out["batched"]["vs_cpu_oracle"] = value / (nom_s / tc)
+
+
+LEGS = ("main", "plain", "small", "high_l", "batched", "serving", "contraction", "cpu")
+
+
+def progress(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    for leg in LEGS[1:]:
+        ap.add_argument("--no-" + leg.replace("_", "-"), dest="no_" + leg, action="store_true")
+    ap.add_argument("--only", choices=LEGS, help="run one leg and print its result (profiling)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the launch-parameter autotuner on the medium cluster before timing")
+    ap.add_argument("--no-autotune", action="store_true", help="the uneven batch with default launch parameters")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gib", type=float, default=32.0)
+    ap.add_argument("--serve-max-batch", type=int, default=4000000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one device copy of 256 MiB (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build evaluates on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding by ECP centre across GPUs is a design, not built)", file=sys.stderr)
+        sys.exit(2)
+    if args.only:
+        for leg in LEGS[1:]:
+            setattr(args, "no_" + leg, leg != args.only)
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    if args.only and args.only not in ("main", "plain", "cpu", "contraction"):
+        out = {}
+    else:
+        out, m, I_gpu = leg_main(args)
+        progress(f"medium cluster {out['ms_per_step']:.3f} ms/step")
+        nominal, evaluated = out["config"]["nominal_triplets"], out["config"]["evaluated_triplets"]
     if not args.no_plain:
-        times = []
-        for _ in range(5):
-            t = time.perf_counter()
-            prc, I_plain = native.get_integrals(m)
-            times.append(time.perf_counter() - t)
-        tp = statistics.median(times)
-        out["plain"] = {"value": nominal / tp, "unit": "shell-triplets/s", "ms_per_call": 1e3 * tp,
-                        "what": "getIntegrals per call: host arrays in, handle + tables built per call, "
-                                "PCIe both ways, upper-triangle matrix out",
-                        "max_abs_vs_resident": float(np.abs(I_plain - I_gpu).max())}
-
-    if not args.no_batch:
-        mols = uneven_batch(0)
-        t = time.perf_counter()
-        bb = native.Batch(mols)
-        t_create = time.perf_counter() - t
-        nom_b, ev_b = bb.triplets()
-        tune_b = bb.autotune(2) if args.autotune else "defaults"
-        bb.execute(2)
-        st_b = bb.stage_ms()
-        t = time.perf_counter()
-        ms_b = bb.execute(5)
-        wb = (time.perf_counter() - t) / 5
-        bb.close()
-        out["batched"] = {"value": nom_b / wb, "unit": "shell-triplets/s", "ms_per_step": 1e3 * wb,
-                          "device_ms_per_step": ms_b, "nominal_triplets": nom_b, "evaluated_triplets": ev_b,
-                          "host_setup_s": t_create, "autotune": tune_b,
-                          "stage_ms_first_step": dict(zip(["tables", "quadratures", "assembly"], st_b[:3])),
-                          "workload": "uneven_batch(seed=0): 256 molecules, 3..40 atoms, 20% ECP atoms "
-                                      "(L=4), basis l <= 3, one resident job"}
+        out["plain"] = leg_plain(m, I_gpu, nominal, evaluated)
+        progress("plain getIntegrals done")
+    ctx = {}
+    if not args.no_small:
+        out["small"] = leg_small(args, ctx)
+        progress("small molecule done")
+    if not args.no_high_l:
+        out["high_l"] = leg_high_l(args, ctx)
+        progress("high-l configuration done")
+    if not args.no_batched:
+        out["batched"] = leg_batched(args, ctx)
+        progress(f"uneven batch {out['batched']['ms_per_step']:.2f} ms/step")
         if "plain" in out:
             out["batched"]["vs_plain"] = out["batched"]["value"] / out["plain"]["value"]
-
-    if not args.no_serve:
+    if not args.no_serving:
         out["serving"] = serve_bench(args.serve_requests, args.serve_slots, args.serve_budget_gib,
                                      args.serve_max_batch)
+        progress("serving done")
         if "plain" in out:
             out["serving"]["vs_plain"] = out["serving"]["value"] / out["plain"]["value"]
-
-    if not args.no_contract:
+    if not args.no_contraction:
         out["contraction"] = contract_bench(m)
-
+        progress("contraction done")
     if not args.no_cpu:
-        cpu, I_ref = cpu_baseline(m, nominal)
+        progress("CPU baselines (the reference on the host's cores) ...")
+        cpu, I_ref = cpu_baseline(m, nominal, evaluated)
         if cpu:
-            single, multi = cpu
-            out["cpu_baseline"] = multi
-            out["cpu_baseline_single_core"] = single
+            main_cpu, more = cpu
+            out["cpu_baseline"] = main_cpu
+            out["cpu_more"] = more
             d = np.abs(I_gpu - I_ref)
             at = int(np.argmax(d))
             out["max_abs_vs_reference"] = float(d.flat[at])
@@ -456,8 +697,12 @@ def main():
             out["elements_above_1e-10_abs"] = int((d > 1e-10).sum())
             # <= 0: every element within 1e-10 absolute + 1e-8 relative
             out["max_excess_over_tolerance"] = float((d - (1e-10 + 1e-8 * np.abs(I_ref))).max())
-            out["speedup_vs_cpu_all_cores"] = value / multi["value"]
-
+            out["speedup_vs_cpu_all_cores"] = out["value"] / main_cpu["value"]
+        progress("CPU checks of the other legs ...")
+        try:
+            cpu_checks(ctx, out)
+        except Exception as e:  # a CPU checker that dies must not take the GPU figures with it
+            out["cpu_checks_error"] = repr(e)
     print(json.dumps(out))
 
 

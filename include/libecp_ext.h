@@ -24,6 +24,20 @@ int libecp_screened_triplets(libECPHandle *h, int *keys, int cap);
    screening (libecp.c:256-312), the unit of the throughput metric. */
 long long libecp_nominal_triplets(libECPHandle *h);
 
+/* Algorithmic FLOPs of the reference's eager algorithm on the handle's
+   screened triplets (host only, no device work), up to n of: F tables,
+   type-1 small grid, type-2 small grid, type-1 assembly, type-2 assembly,
+   binomial shift (the fallbacks depend on which integrals fail, known only
+   after an evaluation: libecp_batch_profile reports them).  FMA = 2,
+   add/sub/mul/div = 1, exp/pow/sqrt/log/trig = 1.  Returns the number of
+   terms (8; the last two, the fallbacks, are 0 here), or -1. */
+int libecp_model_flops(libECPHandle *h, double *out, int n);
+/* The fallback terms of the same model: type 1, a primitive pair of shells
+   la, lb with `failed` failed small-grid entries on a large grid of
+   `points` points; type 2, one primitive pair of a (triplet, channel l)
+   whose channel has ng_l Gaussians, with `failed` failed T entries. */
+double libecp_model_fallback_flops(int type, int la, int lb, int l, int ng_l, int failed, int points);
+
 /* Phases of the last getIntegrals or calculateECPIntegrals call made by the
    calling thread, in ms, up to n of: host tables (libECP_init; ~0 when they
    come from the process cache), screening (triplet list), device context
@@ -136,10 +150,13 @@ int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples);
    for small jobs), "asm2_cache" (type-2 assembly with the angular factors
    in registers: 0 no, 1 yes, 2, the default: with fused launches),
    "t2_head" (PS93 levels of the type-2 small-grid kernel before the entries
-   still open are rerun compacted; 15: no split), "t2_tail_grid",
+   still open are rerun compacted; 15: no split; 0, the default: 6 with
+   fused launches, no split otherwise), "t2_tail_grid",
    "fb2_split" (waves sharing one (triplet, primitive pair) of the type-2
    large-grid fallback; 0, the default: 3 for jobs of up to a million
-   (triplet, pair) items, 1 for larger ones).  Results do
+   (triplet, pair) items, 1 for larger ones).  "t1_spec" (fused launches: 1, the default, starts
+   the type-1 large-grid fallback beside the small-grid tail on every entry
+   the head left open; 0 after the tail on the failed entries).  Results do
    not depend on them.  0, or -1 for an unknown name. */
 int libecp_batch_set_param(libecp_batch *b, const char *name, int value);
 int libecp_batch_get_param(libecp_batch *b, const char *name, int *value);

@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libecp_amd.so")
 
-C_SOURCES = ["ecp_tables.c", "ecp_job.c", "ecp_api.c", "ecp_serve.c", "ecp_io.c"]
+C_SOURCES = ["ecp_tables.c", "ecp_job.c", "ecp_api.c", "ecp_serve.c", "ecp_io.c", "ecp_flops.c"]
 HIP_SOURCES = ["ecp_device.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -488,6 +488,31 @@ int libecp_screened_triplets(libECPHandle *h, int *keys, int cap) {
   return J->ntrip;
 }
 
+int libecp_model_flops(libECPHandle *h, double *out, int n) {
+  char err[512] = "";
+  if (!h || !out) {
+    set_last_error("NULL argument");
+    return -1;
+  }
+  if (h->job_ready) {
+    ecp_job_free(&h->job);
+    h->job_ready = 0;
+  }
+  if (ecp_job_build(&h->mol, 1, h->n, h->st, &h->job, err, sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  h->job_ready = 1;
+  double v[ECP_FL_N];
+  ecp_model_flops(&h->job, v);
+  for (int i = 0; i < ECP_FL_N && i < n; i++) out[i] = v[i];
+  return ECP_FL_N;
+}
+
+double libecp_model_fallback_flops(int type, int la, int lb, int l, int ng_l, int failed, int points) {
+  return type == 1 ? ecp_model_fb1_unit(la + lb, failed, points) : ecp_model_fb2_pair(la, lb, l, ng_l, failed, points);
+}
+
 long long libecp_nominal_triplets(libECPHandle *h) {
   if (!h) return -1;
   long long ne = 0, ns = 0;
@@ -726,6 +751,7 @@ static int *tuning_field(ecp_tuning *t, const char *name) {
   if (!strcmp(name, "t2_head")) return &t->t2_head;
   if (!strcmp(name, "t2_tail_grid")) return &t->t2_tail_grid;
   if (!strcmp(name, "fb2_split")) return &t->fb2_split;
+  if (!strcmp(name, "t1_spec")) return &t->t1_spec;
   return NULL;
 }
 
@@ -789,12 +815,13 @@ int libecp_batch_autotune(libecp_batch *b, int repeats, char *report, int len) {
   } space[] = {
       {"fuse", 3, {0, 1, 2}},
       {"asm2_cache", 3, {0, 1, 2}},
-      {"t2_head", 5, {4, 6, 8, 10, 15}},
+      {"t2_head", 6, {0, 4, 6, 8, 10, 15}},
       {"t1_coop_max", 5, {0, 20000, 100000, 400000, 1 << 30}},
       {"t1_head_coop", 4, {6, 8, 10, 12}},
       {"t1_head_serial", 4, {6, 8, 10, 15}},
       {"t1_tail_grid", 4, {2048, 8192, 32768, 131072}},
       {"fb2_split", 4, {0, 1, 2, 3}},
+      {"t1_spec", 2, {0, 1}},
   };
   for (size_t k = 0; k < sizeof(space) / sizeof(space[0]); k++) {
     for (int i = 0; i < space[k].n; i++) {

@@ -48,6 +48,30 @@ typedef struct {
   int npairs;
 } ecp_schedule;
 
+/* Functions shared by the host model and the device kernels (plain C in
+   the host library, host + device functions in the HIP source). */
+#ifdef __HIPCC__
+#define ECP_HD __host__ __device__
+#else
+#define ECP_HD
+#endif
+
+/* FLOP model of the large-grid fallbacks (the reference's eager algorithm,
+   type1.c:148-196, type2.c:417-528; FMA = 2, libm calls 1):
+   type 1, a primitive pair of class lab with f failed small-grid entries:
+   N points tabulated (exponent, Bessel orders 0..lab, U_L, powers of r)
+   and f integrals over them;
+   type 2, one primitive pair of a (triplet, channel l) group with f failed
+   entries: N points with two Bessel vectors (orders to la + l and lb + l),
+   the exponent and its exp, U_l (ng Gaussians), the powers r^k, and per
+   entry and point the five-factor integrand, its weight and sum. */
+static inline ECP_HD double ecp_fb1_flops(int lab, int f, int N) {
+  return f > 0 ? (double)N * (31.0 * lab + 74) + f * (4.0 * N + 150) : 0.0;
+}
+static inline ECP_HD double ecp_fb2_flops(int la, int lb, int l, int ng, int f, int N) {
+  return f > 0 ? (double)N * (30.0 * (2 * l + la + lb) + 144 + 5.0 * ng + la + lb) + f * (8.0 * N + 150) : 0.0;
+}
+
 /* ---- flat job description (host -> device) ---------------------------- */
 
 typedef struct {

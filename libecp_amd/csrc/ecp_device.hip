@@ -497,7 +497,8 @@ __device__ __forceinline__ void type1_body(int bid, int nb, const DevTables &T, 
                                            unsigned long long *__restrict__ qfail, int *__restrict__ fbq,
                                            int *__restrict__ fbcount, int head, double *__restrict__ state,
                                            size_t plane, int *__restrict__ tq, int *__restrict__ tcount,
-                                           unsigned char *__restrict__ fbflag) {
+                                           unsigned char *__restrict__ fbflag,
+                                           unsigned long long *__restrict__ qopen = nullptr) {
   constexpr int NQ = nq_of(LAB);
   const int count = MODE == 2 ? *tcount : nunits;
   const int stride = MODE == 2 ? nb * 64 : count;
@@ -611,6 +612,8 @@ __device__ __forceinline__ void type1_body(int bid, int nb, const DevTables &T, 
         state[2 * plane + U.qoff + e] = I[e];
       }
       qfail[u] = done;
+      // the entries still open, for a fallback that starts beside the tail
+      if (qopen) qopen[u] = all & ~done;
       tq[atomicAdd(tcount, 1)] = u;
       continue;
     }
@@ -689,7 +692,7 @@ __device__ __forceinline__ void t1tail_body(int bid, int nb, const DevTables &T,
                                             unsigned long long *__restrict__ qfail, int *__restrict__ fbq,
                                             int *__restrict__ fbcount, int head, const double *__restrict__ state,
                                             size_t plane, const int *__restrict__ tq, const int *__restrict__ tcount,
-                                            unsigned char *__restrict__ fbflag, double *s_T) {
+                                            unsigned char *__restrict__ fbflag, double *s_T, bool spec = false) {
   constexpr int NQ = nq_of(LAB);
   static_assert(NQ <= 64, "one lane per Q entry");
   const int count = *tcount;
@@ -796,7 +799,8 @@ __device__ __forceinline__ void t1tail_body(int bid, int nb, const DevTables &T,
     if (lane == 0) {
       qfail[u] = failed;
       if (failed) {
-        fbq[atomicAdd(fbcount, 1)] = u;
+        // spec: the fallback already runs over every unit of the tail
+        if (!spec) fbq[atomicAdd(fbcount, 1)] = u;
         fbflag[U.trip] = 1;
       }
     }
@@ -897,19 +901,37 @@ __device__ __forceinline__ int psm92_round_point(const DevTables &T, int l0, int
 constexpr int kKS = 65;     // LDS stride of per-point Bessel rows (bank spread)
 constexpr int kFb1TS = 33;  // LDS stride of a failed entry's pair sums (at most 32 pairs per call)
 
-template <int LAB>
+template <int LAB, bool SPEC = false>
 __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, const DevJob &J,
                                          const T1Unit *__restrict__ units, double *__restrict__ qout,
                                          const unsigned long long *__restrict__ qfail, const int *__restrict__ fbq,
                                          const int *__restrict__ fbcount, unsigned char *__restrict__ status,
                                          double *s_r, double *s_w, double *s_u, double *s_g, double *s_k,
-                                         double *s_t, int *s_e) {
-  const int count = *fbcount;
+                                         double *s_t, int *s_e, const int *__restrict__ tq = nullptr,
+                                         const int *__restrict__ tcount = nullptr,
+                                         const unsigned long long *__restrict__ qopen = nullptr,
+                                         double *__restrict__ qfb = nullptr,
+                                         unsigned long long *__restrict__ qfbf = nullptr) {
+  // qfb != nullptr (speculative): besides the units whose small-grid
+  // integrals failed (fbq), every unit the tail takes (tq), for the entries
+  // open after the head (qopen), started before the tail has decided them;
+  // results to qfb, large-grid failures as an entry mask to qfbf, and the
+  // assembly uses a result where the small grid failed
+  const int n1 = *fbcount;
+  int count = n1;
+  if constexpr (SPEC) count += *tcount;
   const int lane = threadIdx.x;
   for (int it = bid; it < count; it += nb) {
-    const int u = fbq[it];
+    int u;
+    unsigned long long mask;
+    if (!SPEC || it < n1) {
+      u = fbq[it];
+      mask = qfail[u];
+    } else {
+      u = tq[it - n1];
+      mask = qopen[u];
+    }
     const T1Unit U = units[u];
-    const unsigned long long mask = qfail[u];
     const ecp_triplet tr = J.trip[U.trip];
     const ecp_centre Cn = J.cent[tr.centre];
     T1Geom G;
@@ -1043,7 +1065,17 @@ __device__ __forceinline__ void fb1_body(int bid, int nb, const DevTables &T, co
         lv++;
       }
     }
-    if (mine) {
+    if constexpr (SPEC) {
+      if (mine && done) qfb[U.qoff + my_e] = res;
+      // the failed entries as a mask: one bit per lane that owns one
+      unsigned int lo = (mine && !done && my_e < 32) ? (1u << my_e) : 0u;
+      unsigned int hi = (mine && !done && my_e >= 32) ? (1u << (my_e - 32)) : 0u;
+      for (int o = 32; o > 0; o >>= 1) {
+        lo |= (unsigned int)__shfl_xor((int)lo, o, 64);
+        hi |= (unsigned int)__shfl_xor((int)hi, o, 64);
+      }
+      if (lane == 0) qfbf[u] = ((unsigned long long)hi << 32) | lo;
+    } else if (mine) {
       if (done) qout[U.qoff + my_e] = res;
       else status[2 * U.trip] = 1;
     }
@@ -1101,8 +1133,37 @@ __device__ __forceinline__ int t2_entry(const DevTables &T, const DevJob &J, con
   for (int lv = 2; lv < lv_end; lv++) {
     const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
     int cnt = 0;
-    for (int k = T.s93.lvl_begin[lv]; k < T.s93.lvl_begin[lv + 1]; k++) {
-      const int li = T.p93l[k], ri = T.p93r[k];
+    // four point pairs at a time: their loads are issued together (a lane
+    // that runs every level is otherwise one long chain of dependent
+    // reads), both points evaluated whatever the window (the tables hold
+    // every point), the sums T = (0 + f(left)) + f(right) added in order.
+    // The active lanes walk the levels in step, so the pair index is
+    // uniform and the schedule reads are scalar.
+    const int kb = T.s93.lvl_begin[lv], ke = T.s93.lvl_begin[lv + 1];
+    int k = kb;
+    for (; k + 4 <= ke; k += 4) {
+      const int ku = __builtin_amdgcn_readfirstlane(k);
+      double t[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int li = T.p93l[ku + i], ri = T.p93r[ku + i];
+        const double fl = f(li), fr = f(ri);
+        t[i] = 0.0;
+        if (li >= gs) {
+          cnt++;
+          t[i] += fl;
+        }
+        if (ri <= ge) {
+          cnt++;
+          t[i] += fr;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) I += t[i];
+    }
+    for (; k < ke; k++) {
+      const int ku = __builtin_amdgcn_readfirstlane(k);
+      const int li = T.p93l[ku], ri = T.p93r[ku];
       double t = 0.0;
       if (li >= gs) {
         cnt++;
@@ -1555,6 +1616,10 @@ struct T1Set {
   unsigned char *stat, *fbflag;
   int ubeg[ECP_MAX_LAB + 1], ucnt[ECP_MAX_LAB + 1];
   int head[ECP_MAX_LAB + 1];  // PS93 levels of the head per class (>= the level count: no tail)
+  // speculative fallback (nullptr: off): open entries after the head, the
+  // fallback's results and its failed-entry masks
+  unsigned long long *qopen, *qfbf;
+  double *qfb;
 };
 
 #define ECP_CASES_LAB(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
@@ -1570,7 +1635,7 @@ __global__ __launch_bounds__(64) void k_type1_fused(DevTables T, DevJob J, T1Set
   case L:                                                                                                      \
     if constexpr (L <= MAXLAB)                                                                                 \
       type1_body<L, MODE>(bid, nb, T, J, S.units + ub, S.ucnt[c], S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.head[c], \
-                          S.st, S.plane, S.tq + ub, S.fbc + 32 + c, S.fbflag);                                  \
+                          S.st, S.plane, S.tq + ub, S.fbc + 32 + c, S.fbflag, S.qopen ? S.qopen + ub : nullptr); \
     break;
     ECP_CASES_LAB(X)
 #undef X
@@ -1588,14 +1653,14 @@ __global__ __launch_bounds__(64) void k_t1tail_fused(DevTables T, DevJob J, T1Se
   case L:                                                                                                      \
     if constexpr (L <= MAXLAB)                                                                                 \
       t1tail_body<L>(bid, nb, T, J, S.units + ub, S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.head[c], S.st, S.plane, \
-                     S.tq + ub, S.fbc + 32 + c, S.fbflag, s_T);                                                 \
+                     S.tq + ub, S.fbc + 32 + c, S.fbflag, s_T, S.qfb != nullptr);                                \
     break;
     ECP_CASES_LAB(X)
 #undef X
   }
 }
 
-template <int MAXLAB>
+template <int MAXLAB, bool SPEC>
 __global__ __launch_bounds__(64) void k_fb1_fused(DevTables T, DevJob J, T1Set S, ClassMap cm) {
   __shared__ double s_r[64], s_w[64], s_u[64], s_g[64], s_k[(MAXLAB + 1) * kKS], s_t[nq_of(MAXLAB) * kFb1TS];
   __shared__ int s_e[64];
@@ -1606,8 +1671,9 @@ __global__ __launch_bounds__(64) void k_fb1_fused(DevTables T, DevJob J, T1Set S
 #define X(L)                                                                                                   \
   case L:                                                                                                      \
     if constexpr (L <= MAXLAB)                                                                                 \
-      fb1_body<L>(bid, nb, T, J, S.units + ub, S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.stat, s_r, s_w, s_u, s_g, \
-                  s_k, s_t, s_e);                                                                               \
+      fb1_body<L, SPEC>(bid, nb, T, J, S.units + ub, S.q, S.qf + ub, S.fbq + ub, S.fbc + c, S.stat, s_r, s_w, s_u, \
+                        s_g, s_k, s_t, s_e, S.tq + ub, S.fbc + 32 + c, SPEC ? S.qopen + ub : nullptr, S.qfb,      \
+                        SPEC ? S.qfbf + ub : nullptr);                                                          \
     break;
     ECP_CASES_LAB(X)
 #undef X
@@ -1670,6 +1736,11 @@ struct AsmArgs {
   int lds_S1, lds_S2, shift_lm;  // shift factor tables (aliasing rsph/Q resp. T) for shells up to shift_lm
   int lds_dir;                    // type 1: the pair directions of a chunk (4 values each)
   int lds_group;                  // packed type-1 assembly: LDS doubles per triplet
+  // speculative type-1 fallback (qfb != nullptr): an entry whose small grid
+  // failed (qfail bit) takes the fallback's result qfb; the triplet failed
+  // where the fallback failed too (qfbf bit)
+  const unsigned long long *qfail, *qfbf;
+  const double *qfb;
 };
 
 // binomial shift of a block over C-centred monomials to the IJK(la) x IJK(lb)
@@ -1839,9 +1910,12 @@ __device__ __forceinline__ double asm1_monomial(double acc, const double *__rest
 // G triplets per wave, kAsmThreads / G lanes and an LDS slice of
 // A.lds_group doubles each; the wave's lane groups run the same phases and
 // synchronise together, an inactive group with empty loops.
-template <int G = 1>
+// MAXD: the largest monomial degree whose poly2sph row goes to registers
+// (the triplets' la + lb when a launch covers one class); higher degrees
+// take the general loop.
+template <int G = 1, int MAXD = 6>
 __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, AsmArgs A,
-                                                      const unsigned char *__restrict__ status,
+                                                      unsigned char *__restrict__ status,
                                                       const int *__restrict__ tlist, int ntl,
                                                       const unsigned char *__restrict__ fbflag, int filter) {
   extern __shared__ double lds_all[];
@@ -1872,6 +1946,21 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
   const ecp_centre Cn = J.cent[tr.centre];
   const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
   const int la = tr.la, lb = tr.lb, lab = la + lb;  // shifted for derivatives
+  if constexpr (G == 1) {
+    if (A.qfb) {
+      // speculative fallback: an integral failed where the small grid and
+      // the large grid both failed (type1.c:182-190, code 1)
+      bool bad = false;
+      for (int k = lid; k < Sa.K * Sb.K; k += W) {
+        const int ug = A.t1first[t] + k;
+        bad = bad || (A.qfail[ug] & A.qfbf[ug]) != 0ull;
+      }
+      if (wave_any(bad)) {
+        if (lid == 0) status[2 * t] = 1;
+        return;
+      }
+    }
+  }
   const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
   const double rAC[3] = {ra.x - rc.x, ra.y - rc.y, ra.z - rc.z};
   const double rBC[3] = {rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
@@ -1915,7 +2004,9 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
     }
     for (int it = lid; it < nc * nql; it += W) {
       const int k = it / nql, e = it - k * nql;
-      Qs[k * nql + e] = A.qout[A.units1[u0 + c0 + k].qoff + e];
+      const int ug = u0 + c0 + k;
+      const int qo = A.units1[ug].qoff + e;
+      Qs[k * nql + e] = (A.qfb && ((A.qfail[ug] >> e) & 1ull)) ? A.qfb[qo] : A.qout[qo];
     }
     __syncthreads();
     for (int p = lid; p < np; p += W) {
@@ -1925,10 +2016,12 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
       for (int l1 = 0; l1 < deg; l1++) qstart += l1 / 2 + 1;
       const double *pm = T.poly2sph + (size_t)p * tmd;
       double acc = H[p];
-      if (deg <= 6) {
+      if (deg <= MAXD) {
         switch (deg) {
-#define MONO(D) \
-  case D: acc = asm1_monomial<D>(acc, pm, rsph, A.lds_rs_stride, Qs + qstart, nql, nc); break;
+#define MONO(D)                                                                       \
+  case D:                                                                             \
+    if constexpr (D <= MAXD) acc = asm1_monomial<D>(acc, pm, rsph, A.lds_rs_stride, Qs + qstart, nql, nc); \
+    break;
           MONO(0) MONO(1) MONO(2) MONO(3) MONO(4) MONO(5) MONO(6)
 #undef MONO
         }
@@ -2123,7 +2216,7 @@ struct JobLayout {
          i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, i_tcls2 = 0,
          in_bytes = 0;
   size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_tclaim = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
-         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
+         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_qfb = 0, d_qopen = 0, d_qfbf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0, deriv = 0;
 };
 
@@ -2144,6 +2237,8 @@ struct Prepared {
   double *t1s = nullptr;  // type-1 head -> tail state: p, q, I planes of nq doubles
   int *tq1 = nullptr;     // type-1 tail queues (per class, indexed like the units)
   unsigned long long *qf = nullptr;
+  unsigned long long *qopen = nullptr, *qfbf = nullptr;  // speculative type-1 fallback
+  double *qfb = nullptr;
   int *fbq1 = nullptr, *fbc = nullptr;
   unsigned char *stat = nullptr;
   unsigned char *t1fb = nullptr, *t2fb = nullptr;  // per triplet: has large-grid work (type 1 / type 2)
@@ -2176,6 +2271,7 @@ struct Prepared {
   // eager algorithm): type-1 radial work per class, type-2 radial work
   double flops_t1[ECP_MAX_LAB + 1] = {0};
   double flops_t2 = 0;
+  double model[ECP_FL_N] = {0};  // the model's terms known before evaluation (ecp_model_flops)
 };
 
 }  // namespace
@@ -2187,6 +2283,7 @@ struct ecp_dev {
   hipStream_t stream = nullptr;
   hipStream_t workers[kWorkers] = {};
   hipEvent_t fork = nullptr, t2done = nullptr, t1done = nullptr, angdone = nullptr, join[kWorkers] = {};
+  hipEvent_t h1done = nullptr;  // type-1 head finished (speculative fallback)
   // the type-1 small-grid quadrature (head and tail launches), recorded on
   // its stream: event pair k belongs to captured graph k (replays cycle over
   // the graphs, so the last kSpanGraphs replays can be read); the eager
@@ -2214,7 +2311,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 15, 100000, 8192, 2, 2, 15, 2048, 0};
+  ecp_tuning tune{8, 15, 100000, 8192, 2, 2, 0, 2048, 0, 1};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph[kSpanGraphs] = {};
   hipGraphExec_t gexec[kSpanGraphs] = {};
@@ -2230,6 +2327,7 @@ struct ecp_dev {
   hipEvent_t con_ev[2] = {};
   std::vector<char> con_idx;  // index tables last uploaded to con_mem at con_idx_at
   size_t con_idx_at = 0;
+  double *fbf = nullptr;      // the profile's fallback FLOP sums
 };
 
 namespace {
@@ -2332,6 +2430,7 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
   if ((e = hipEventCreateWithFlags(&d->t2done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->t1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->angdone, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
+  if ((e = hipEventCreateWithFlags(&d->h1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   for (int k = 0; k < kSpanGraphs; k++)
     for (int i = 0; i < 2; i++)
       if ((e = hipEventCreate(&d->span[k][i])) != hipSuccess) return fail("hipEventCreate", e);
@@ -2429,6 +2528,7 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   if (d->arena) (void)hipFree(d->arena);
   if (d->pin_in) (void)hipHostFree(d->pin_in);
   if (d->pin_out) (void)hipHostFree(d->pin_out);
+  if (d->fbf) (void)hipFree(d->fbf);
   for (int i = 0; i < 8; i++)
     if (d->ev[i]) (void)hipEventDestroy(d->ev[i]);
   for (int i = 0; i < kWorkers; i++) {
@@ -2440,6 +2540,7 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   if (d->t2done) (void)hipEventDestroy(d->t2done);
   if (d->t1done) (void)hipEventDestroy(d->t1done);
   if (d->angdone) (void)hipEventDestroy(d->angdone);
+  if (d->h1done) (void)hipEventDestroy(d->h1done);
   for (int k = 0; k < kSpanGraphs; k++)
     for (int i = 0; i < 2; i++)
       if (d->span[k][i]) (void)hipEventDestroy(d->span[k][i]);
@@ -2469,12 +2570,14 @@ namespace {
 // waves on the device) or the compacted lane-serial tail when it is large.
 // type-2 small grid: the lane-per-entry kernel, split into head and
 // compacted tail when t2_head is below the level count
-void launch_type2(ecp_dev *d, hipStream_t s) {
+void launch_type2(ecp_dev *d, hipStream_t s, bool fused) {
   const Prepared &P = d->P;
   if (P.ntot2 <= 0) return;
   T2Args A{P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + kFb2Cnt, P.nt,
            P.t2fb, P.flist, P.tq2, P.fbc + 48, P.tclaim, P.tcls2};
-  const int head = d->tune.t2_head;
+  // 0: automatic -- a head of 6 levels when the small grid is on the
+  // critical chain (fused launches), none in large per-class jobs
+  const int head = d->tune.t2_head > 0 ? d->tune.t2_head : (fused ? 6 : 15);
   {
     Span sp(d, s, FAM_TYPE2);
     k_type2<<<(P.ntot2 + 127) / 128, 128, 0, s>>>(d->T, P.J, A, head);
@@ -2507,14 +2610,34 @@ void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache) {
 // wave, or with pack those of classes la + lb <= kAsmPackLab kAsmPack to a
 // wave (one more launch: it pays in large per-class launches, not on the
 // critical chain of fused ones)
-void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack) {
+// cls >= 0: every triplet of the range is of class la + lb = cls (per-class
+// launches), which bounds the monomial degrees the kernel holds in registers
+// spec: the type-1 fallback ran speculatively (T1Set::qfb), the assembly
+// picks its results where the small grid failed
+void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack, int cls = -1, bool spec = false) {
   const Prepared &P = d->P;
+  AsmArgs aa = P.aa;
+  if (spec) {
+    aa.qfail = P.qf;
+    aa.qfbf = P.qfbf;
+    aa.qfb = P.qfb;
+  }
   const int m = pack ? std::min(std::max(b, P.tcls_begin[kAsmPackLab + 1]), e) : b;
   if (m > b)
-    k_asm1<kAsmPack><<<(m - b + kAsmPack - 1) / kAsmPack, kAsmThreads, P.asm1_lds4, s>>>(
+    k_asm1<kAsmPack, kAsmPackLab><<<(m - b + kAsmPack - 1) / kAsmPack, kAsmThreads, P.asm1_lds4, s>>>(
         d->T, P.J, P.aa4, P.stat, P.tcls + b, m - b, P.t1fb, filter);
-  if (e > m)
-    k_asm1<1><<<e - m, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, P.aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
+  if (e <= m) return;
+  const dim3 g(e - m);
+  switch (cls) {
+#define ASM1(D)                                                                                               \
+  case D:                                                                                                   \
+    k_asm1<1, D><<<g, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, aa, P.stat, P.tcls + m, e - m, P.t1fb, filter); \
+    break;
+    ASM1(0) ASM1(1) ASM1(2) ASM1(3) ASM1(4) ASM1(5)
+#undef ASM1
+    default:
+      k_asm1<1, 6><<<g, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
+  }
 }
 
 // The assembly of the triplets with large-grid work (filter 2) comes last
@@ -2626,6 +2749,31 @@ void launch_ang(ecp_dev *d, hipStream_t s) {
 // primitive pairs (latency-bound: per-class launches hold few waves each)
 constexpr int kFuseAutoMaxUnits = 150000;
 
+// one type-1 class of the per-class sequence: small grid, fallback, assembly
+template <int c>
+void enqueue_t1_class_t(ecp_dev *d, hipStream_t ws) {
+  const Prepared &P = d->P;
+  launch_type1<c>(d, ws, d->T, P.J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],
+                  P.fbq1 + P.cls_begin[c], P.fbc + c, P.t1s, P.nq, P.tq1 + P.cls_begin[c], P.fbc + 32 + c, P.t1fb);
+  launch_fb1<c>(d, ws, d->T, P.J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],
+                P.fbq1 + P.cls_begin[c], P.fbc + c, P.stat);
+  if (P.tcls_begin[c + 1] > P.tcls_begin[c]) {
+    Span sp(d, ws, FAM_ASM);
+    launch_asm1(d, ws, P.tcls_begin[c], P.tcls_begin[c + 1], 0, true, c);
+  }
+}
+
+void enqueue_t1_class(ecp_dev *d, hipStream_t ws, int c) {
+  switch (c) {
+#define T1(c) \
+  case c:     \
+    enqueue_t1_class_t<c>(d, ws); \
+    break;
+    T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10)
+#undef T1
+  }
+}
+
 void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   const int units = d->P.cls_begin[ECP_MAX_LAB + 1];
   if (d->tune.fuse == 1 || (d->tune.fuse == 2 && units <= kFuseAutoMaxUnits)) {
@@ -2654,7 +2802,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     launch_ang(d, s);
   }
   if (P.ntot2 > 0) {
-    launch_type2(d, w2);
+    launch_type2(d, w2, false);
     (void)hipEventRecord(d->t2done, w2);
     (void)hipStreamWaitEvent(w2, d->angdone, 0);
     {
@@ -2672,23 +2820,22 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       launch_asm2(d, s, 2, false);
     }
   }
-  // type 1 classes, each followed by its fallback and assembly, round-robin
-  // over the other workers
-#define T1(c)                                                                                                   \
-  {                                                                                                             \
-    hipStream_t ws = w[(c) % (kWorkers - 1)];                                                                   \
-    launch_type1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                \
-                    P.fbq1 + P.cls_begin[c], P.fbc + c, P.t1s, P.nq, P.tq1 + P.cls_begin[c],                     \
-                    P.fbc + 32 + c, P.t1fb);                                                                    \
-    launch_fb1<c>(d, ws, T, J, P.u1 + P.cls_begin[c], P.cls_count[c], P.q, P.qf + P.cls_begin[c],                  \
-                  P.fbq1 + P.cls_begin[c], P.fbc + c, P.stat);                                                  \
-    if (P.tcls_begin[(c) + 1] > P.tcls_begin[c]) {                                                            \
-      Span sp(d, ws, FAM_ASM);                                                                                  \
-      launch_asm1(d, ws, P.tcls_begin[c], P.tcls_begin[(c) + 1], 0, true);                                      \
-    }                                                                                                           \
+  // type-1 classes, each followed by its fallback and assembly on one of the
+  // other workers: the costliest class first, each on the worker with the
+  // least work so far (the cost of a class: its algorithmic small-grid
+  // FLOPs), so that no large class starts last and runs alone at the end
+  int order[ECP_MAX_LAB + 1], nord = 0;
+  for (int c = 0; c <= ECP_MAX_LAB; c++)
+    if (P.cls_count[c] > 0) order[nord++] = c;
+  std::stable_sort(order, order + nord, [&](int a, int b) { return P.flops_t1[a] > P.flops_t1[b]; });
+  double load[kWorkers - 1] = {};
+  for (int i = 0; i < nord; i++) {
+    int k = 0;
+    for (int j = 1; j < kWorkers - 1; j++)
+      if (load[j] < load[k]) k = j;
+    load[k] += P.flops_t1[order[i]];
+    enqueue_t1_class(d, w[k], order[i]);
   }
-  T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10)
-#undef T1
   for (int i = 0; i < kWorkers; i++) {
     (void)hipEventRecord(d->join[i], w[i]);
     (void)hipStreamWaitEvent(s, d->join[i], 0);
@@ -2762,6 +2909,16 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     tail = tail || (coop && P.cls_count[c] > 0);
   }
   const bool big1 = P.max_lab > 6;
+  // speculative fallback: with a tail, the large-grid fallback starts after
+  // the head on every entry the head left open, beside the tail, instead of
+  // after the tail on the failed ones (most of the entries open after the
+  // head fail; the tail only decides which results the assembly takes)
+  const bool spec = tail && tu.t1_spec != 0;
+  if (spec) {
+    S.qopen = P.qopen;
+    S.qfb = P.qfb;
+    S.qfbf = P.qfbf;
+  }
   const ClassMap cmh = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return (P.cls_count[c] + 63) / 64; });
   if (cmh.n > 0) {
     // external event nodes when captured, so that replays record them too
@@ -2771,6 +2928,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       if (big1) k_type1_fused<10, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
       else k_type1_fused<6, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
     }
+    if (spec) (void)hipEventRecord(d->h1done, w[0]);
     if (tail) {
       const ClassMap cmt = class_map(ECP_MAX_LAB + 1, tailcnt,
                                      [&](int c) { return std::min(tailcnt[c], tu.t1_tail_grid); });
@@ -2784,20 +2942,25 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     // continues a node's queue with its first dependent, and the fallback
     // is the critical one
     (void)hipEventRecord(d->t1done, w[0]);
-    (void)hipStreamWaitEvent(w[1], d->t1done, 0);
+    (void)hipStreamWaitEvent(w[1], spec ? d->h1done : d->t1done, 0);
     const ClassMap cmf = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return std::min(P.cls_count[c], 4096); });
     {
       Span sp(d, w[1], FAM_FB1);
-      if (big1) k_fb1_fused<10><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
-      else k_fb1_fused<6><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      if (big1 && spec) k_fb1_fused<10, true><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      else if (big1) k_fb1_fused<10, false><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      else if (spec) k_fb1_fused<6, true><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      else k_fb1_fused<6, false><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
     }
     {
       Span sp(d, w[0], FAM_ASM);
       launch_asm1(d, w[0], 0, P.nt, 1, false);
     }
+    // the assembly of the triplets with large-grid work reads the tail's
+    // decisions and the fallback's results
+    if (spec) (void)hipStreamWaitEvent(w[1], d->t1done, 0);
     {
       Span sp(d, w[1], FAM_ASM);
-      launch_asm1(d, w[1], 0, P.nt, 2, false);
+      launch_asm1(d, w[1], 0, P.nt, 2, false, -1, spec);
     }
   }
   // ---- type 2 ----
@@ -2808,7 +2971,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (P.ntot2 > 0) {
     // likewise the fallback first (it needs no angular tables), then the
     // assembly beside it, then the assembly that waits for the fallback
-    launch_type2(d, w[2]);
+    launch_type2(d, w[2], true);
     (void)hipEventRecord(d->t2done, w[2]);
     (void)hipStreamWaitEvent(w[3], d->t2done, 0);
     const int nsplit = fb2_split(d, true);
@@ -2927,6 +3090,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   }
   P.ngroups = (int)groups.size();
   P.ntot2 = ntot2;
+  ecp_model_flops(job, P.model);
   for (int c = 0; c <= ECP_MAX_LAB; c++)
     if (P.cls_count[c]) P.max_lab = c;
   for (int c = 0; c <= kMaxLMB; c++)
@@ -3001,6 +3165,9 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_blk = dv_take(8 * job->nblk);
   Ly.d_t1s = dv_take(3 * 8 * nq);
   Ly.d_tq1 = dv_take(4 * units1.size());
+  Ly.d_qfb = dv_take(8 * nq);
+  Ly.d_qopen = dv_take(8 * units1.size());
+  Ly.d_qfbf = dv_take(8 * units1.size());
   Ly.total = dv_off;
   Ly.FL = job->FL;
   Ly.UK = job->UK;
@@ -3162,6 +3329,9 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.t1s = (double *)(A + Ly.d_t1s);
   P.tq1 = (int *)(A + Ly.d_tq1);
   P.qf = (unsigned long long *)(A + Ly.d_qf);
+  P.qfb = (double *)(A + Ly.d_qfb);
+  P.qopen = (unsigned long long *)(A + Ly.d_qopen);
+  P.qfbf = (unsigned long long *)(A + Ly.d_qfbf);
   P.fbq1 = (int *)(A + Ly.d_fbq1);
   P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 fallback queues, [16..27] type-2, [32..42] type-1 tails,
                                   // [48] type-2 tail
@@ -3289,6 +3459,39 @@ extern "C" int ecp_dev_execute(ecp_dev *d, int repeats, double *ms_total, char *
 
 // One eager evaluation with an event pair around every kernel launch on its
 // own stream; per family: launches, summed duration (ms), algorithmic FLOPs.
+namespace {
+// Algorithmic FLOPs of the large-grid fallbacks of the last evaluation
+// (ecp_fb1_flops / ecp_fb2_flops per failed primitive pair / group): every
+// term is an integer below 2^53, so the atomic sum is exact in any order.
+__global__ __launch_bounds__(256) void k_fb_flops(DevJob J, const T1Unit *__restrict__ units,
+                                                  const unsigned long long *__restrict__ qf, int nunits,
+                                                  const T2Group *__restrict__ groups, const int *__restrict__ gflag,
+                                                  int ngroups, int N, double *out) {
+  double s1 = 0.0, s2 = 0.0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < nunits; u += stride) {
+    const int f = __popcll(qf[u]);
+    if (!f) continue;
+    const ecp_triplet tr = J.trip[units[u].trip];
+    s1 += ecp_fb1_flops(tr.la + tr.lb, f, N);
+  }
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    const int f = gflag[g];
+    if (!f) continue;
+    const T2Group G = groups[g];
+    const ecp_triplet tr = J.trip[G.trip];
+    const ecp_centre C = J.cent[tr.centre];
+    int ng = 0;
+    for (int i = 0; i < C.ng; i++) ng += J.gauss[C.g0 + i].l == G.l;
+    const int KK = J.shells[tr.sa].K * J.shells[tr.sb].K;
+    s2 += KK * ecp_fb2_flops(tr.la, tr.lb, G.l, ng, f, N);
+  }
+  if (s1 != 0.0) atomicAdd(&out[0], s1);
+  if (s2 != 0.0) atomicAdd(&out[1], s2);
+}
+
+}  // namespace
+
 extern "C" int ecp_dev_profile(ecp_dev *d, int *launches, double *ms, double *flops, char *err, size_t errlen) {
   if (!d->prepared) {
     snprintf(err, errlen, "ecp_dev_profile before ecp_dev_prepare");
@@ -3315,8 +3518,24 @@ extern "C" int ecp_dev_profile(ecp_dev *d, int *launches, double *ms, double *fl
     launches[d->pfam[i]]++;
     ms[d->pfam[i]] += t;
   }
-  for (int c = 0; c <= ECP_MAX_LAB; c++) flops[FAM_TYPE1] += d->P.flops_t1[c];
-  flops[FAM_TYPE2] = d->P.flops_t2;
+  // algorithmic FLOPs per family: the model's terms known before the
+  // evaluation, and the fallbacks from this evaluation's failed integrals
+  const Prepared &P = d->P;
+  flops[FAM_TABLES] = P.model[ECP_FL_FTAB];
+  flops[FAM_TYPE1] = P.model[ECP_FL_T1];
+  flops[FAM_TYPE2] = P.model[ECP_FL_T2];
+  flops[FAM_ASM] = P.model[ECP_FL_ASM1] + P.model[ECP_FL_ASM2] + P.model[ECP_FL_SHIFT];
+  if (!d->fbf) ECP_CHECK(hipMalloc(&d->fbf, 2 * sizeof(double)));
+  ECP_CHECK(hipMemsetAsync(d->fbf, 0, 2 * sizeof(double), d->stream));
+  const int nunits = P.cls_begin[ECP_MAX_LAB + 1];
+  const int nb = std::max(1, std::min(1024, (std::max(nunits, P.ngroups) + 255) / 256));
+  k_fb_flops<<<nb, 256, 0, d->stream>>>(P.J, P.u1, P.qf, nunits, P.grp, P.gflag, P.ngroups, d->T.lorder, d->fbf);
+  ECP_CHECK(hipGetLastError());
+  double fb[2] = {0.0, 0.0};
+  ECP_CHECK(hipMemcpyAsync(fb, d->fbf, sizeof(fb), hipMemcpyDeviceToHost, d->stream));
+  ECP_CHECK(hipStreamSynchronize(d->stream));
+  flops[FAM_FB1] = fb[0];
+  flops[FAM_FB2] = fb[1];
   return FAM_N;
 }
 

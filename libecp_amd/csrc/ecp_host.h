@@ -82,10 +82,13 @@ typedef struct {
   int asm2_cache;     /* type-2 assembly with angular factors in registers: 0 no, 1 yes,
                          2 with fused launches only */
   int t2_head;        /* PS93 levels of the type-2 head kernel before the compacted tail
-                         (>= the level count: no tail) */
+                         (>= the level count: no tail; 0: 6 with fused launches, else none) */
   int t2_tail_grid;   /* most workgroups of the type-2 tail launch */
   int fb2_split;      /* waves per (triplet, primitive pair) of the type-2 fallback, its 64-entry
                          passes dealt round robin; 0: 3 for small jobs, 1 for large ones */
+  int t1_spec;        /* fused launches: the type-1 large-grid fallback starts after the head,
+                         beside the tail, on every entry the head left open (1), or after the
+                         tail on the failed ones only (0) */
 } ecp_tuning;
 
 /* ---- device layer (ecp_device.hip) ---- */
@@ -175,6 +178,23 @@ void ecp_mol_release(ecp_mol *m);
 void ecp_job_ao_offsets(const ecp_job *J, int *aoff);
 int ecp_job_scatter(const ecp_job *J, int m, const int *aoff, const double *blocks, const unsigned char *status,
                     int rowdim, int packed_n, double *I);
+
+/* ---- algorithmic FLOP model (ecp_flops.c) ---- */
+enum { ECP_FL_FTAB = 0, ECP_FL_T1, ECP_FL_T2, ECP_FL_ASM1, ECP_FL_ASM2, ECP_FL_SHIFT, ECP_FL_FB1, ECP_FL_FB2,
+       ECP_FL_N };
+/* the terms known before evaluation (all but the fallbacks) of a job */
+void ecp_model_flops(const ecp_job *J, double *out /* [ECP_FL_N] */);
+double ecp_model_t1_unit(int lab, int G);
+double ecp_model_asm1_unit(int la, int lb);
+double ecp_model_t2_group(int la, int lb, int l, int G);
+double ecp_model_asm2_group(int la, int lb, int l);
+double ecp_model_shift(int la, int lb);
+double ecp_model_ftab(int K, int G, int m);
+/* fallbacks: a primitive pair of class lab with f failed entries, N points;
+   a (triplet, l) group with f failed entries, per primitive pair */
+double ecp_model_fb1_unit(int lab, int f, int N);
+double ecp_model_fb2_pair(int la, int lb, int l, int ng_l, int f, int N);
+int ecp_model_gauss_of(const ecp_job *J, int centre, int l);
 
 /* ---- host helpers (ecp_job.c) ---- */
 double ecp_eval_ecp_host(const ecp_gauss *g, int ng, int l, double r);

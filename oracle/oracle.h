@@ -77,6 +77,12 @@ int orc_getIntegrals(int nrAtoms, const double *geometry,
                      int largeGridOrder, double tolerance, double accuracy,
                      int rowdim, double *I, int options);
 
+/* Op-counting build only (liboracle_ops.so): FP64 operations executed,
+   per phase of the algorithm (ORC_PH_* in oracle_internal.h), since the
+   last reset. */
+void orc_ops_reset(void);
+int orc_ops_get(long long *out, int n);
+
 /* Table-level entry points used by the unit tests. */
 void orc_grid_ps93(int maxPoints, double *x, double *w, int *order);   /* KK-mapped */
 void orc_grid_psm92(int maxPoints, double *x, double *w, int *order);  /* on (-1,1) */

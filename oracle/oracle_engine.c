@@ -135,11 +135,11 @@ static arr_t usp_table(int l, const double *r) {
   arr_t t = arr_new(dim, dim, dim, 1);
   double x = 1.0, y = 1.0, z = 1.0;
   for (int i = 0; i <= l; i++) {
-    x = (0 == i) ? 1.0 : -x * r[0];
+    x = (0 == i) ? (double)1.0 : -x * r[0];
     for (int j = 0; j <= l - i; j++) {
-      y = (0 == j) ? 1.0 : -y * r[1];
+      y = (0 == j) ? (double)1.0 : -y * r[1];
       for (int k = 0; k <= l - i - j; k++) {
-        z = (0 == k) ? 1.0 : -z * r[2];
+        z = (0 == k) ? (double)1.0 : -z * r[2];
         t.a[i * dim * dim + j * dim + k] = x * y * z;
       }
     }
@@ -148,7 +148,7 @@ static arr_t usp_table(int l, const double *r) {
 }
 
 static double binom(int n, int k, const double *fac) {
-  return (k >= 0 && k <= n) ? fac[n] / (fac[n - k] * fac[k]) : 0.0;
+  return (k >= 0 && k <= n) ? fac[n] / (fac[n - k] * fac[k]) : (double)0.0;
 }
 
 /* Two-stage binomial shift of a block over C-centred monomials to the A-
@@ -162,6 +162,7 @@ static double *shift_block(const orc_handle *h, const double *gamma, int gcols, 
   const int incJ = ODIM_C(lb), incI = ODIM_IJK(lb);
   double *J = calloc((size_t)ODIM_IJK(la) * ODIM_C(lb), sizeof(double));
   double *I = calloc((size_t)ODIM_IJK(la) * ODIM_IJK(lb), sizeof(double));
+  ORC_PHASE(ORC_PH_SHIFT);
   for (int c1 = 0; c1 < ODIM_IJK(la); c1++) {
     int p = 3 * OIDX_C(la, c1);
     int ax = ijk[p], ay = ijk[p + 1], az = ijk[p + 2];
@@ -241,6 +242,7 @@ static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, 
   double *K = calloc((size_t)(lab + 1) * lo, sizeof(double));
   double *rn = calloc((size_t)(lab + 1) * lo, sizeof(double));
   q_params ps;
+  ORC_PHASE(ORC_PH_T1);
   ps.C = ca * cb * exp(zd2);
   ps.scale = (h->options & ORC_OPT_PERTURB_T1) ? 1.0 + ldexp(1.0, -50) : 1.0;
   ps.expo = expo;
@@ -272,6 +274,7 @@ static int calc_q(orc_handle *h, const double *U_L, const orc_ecp *U, double s, 
     }
   }
   if (nFailed > 0) {
+    ORC_PHASE(ORC_PH_FB1);
     orc_grid *g = orc_grid_copy(h->largeGrid);
     double zeta_p = za + zb;
     double P = (za * dAC + zb * dBC) / zeta_p;
@@ -345,6 +348,7 @@ static double *calc_chi(orc_handle *h, const double *U_L, const orc_ecp *U, cons
     for (int pb = 0; pb < Nb; pb++) {
       double zb = zb_[pb], cb = cb_[pb];
       double P[3], S[3];
+      ORC_PHASE(ORC_PH_ASM1);
       for (int p = 0; p < shiftb; p++) cb *= zb;
       for (int p = 0; p < 3; p++) P[p] = 2.0 * (za * rAC[p] + zb * rBC[p]);
       orc_sph_coords(P, S);
@@ -356,6 +360,7 @@ static double *calc_chi(orc_handle *h, const double *U_L, const orc_ecp *U, cons
         free(Q);
         return NULL;
       }
+      ORC_PHASE(ORC_PH_ASM1);
       for (int ax = 0; ax <= la; ax++)
         for (int ay = 0; ay <= la - ax; ay++)
           for (int az = 0; az <= la - ax - ay; az++) {
@@ -422,6 +427,7 @@ static arr_t tab_ecp(const orc_ecp *U, orc_grid *g, int lmax, double tol) {
   arr_t T = arr_new(U->L, lmax + 1, g->order, 1);
   const int inc1 = T.d[2], inc2 = T.d[1] * T.d[2];
   double *Ul = calloc((size_t)g->order, sizeof(double));
+  ORC_PHASE(ORC_PH_UTAB);
   for (int l = 0; l < U->L; l++) {
     for (int n = 0; n < g->order; n++) Ul[n] = orc_eval_ecp(U, l, g->x[n]);
     g->end = potential_screening(g, Ul, tol);
@@ -450,6 +456,7 @@ static arr_t calc_f(orc_handle *h, const double *rC, screen_t **psg, int *err) {
   const int inc1 = F.d[3], inc2 = F.d[2] * F.d[3], inc3 = F.d[1] * F.d[2] * F.d[3];
   double *K = calloc((size_t)h->maxLambda + 1, sizeof(double));
   int offset = 0, index = 0;
+  ORC_PHASE(ORC_PH_FTAB);
   *psg = sg;
   *err = 0;
   for (int A = 0; A < nAt; A++) {
@@ -537,6 +544,7 @@ static int calc_t_fallback(orc_handle *h, double *T, int Tinc1, int Tinc2, const
   const int laC = alpha + l, lbC = beta + l, lab = alpha + beta;
   const int lo = h->largeGrid->order;
   int rc = 0;
+  ORC_PHASE(ORC_PH_FB2);
   double *expo = calloc((size_t)lo, sizeof(double));
   double *Ub = calloc((size_t)lo, sizeof(double));
   double *Ka = calloc((size_t)(laC + 1) * lo, sizeof(double));
@@ -625,6 +633,7 @@ static double *calc_gamma(orc_handle *h, const arr_t *F, const arr_t *UT, const 
     double *T = calloc((size_t)(laC + 1) * (lbC + 1) * (labn + 1), sizeof(double));
     int *fl = calloc(3 * (size_t)(laC + 1) * (lbC + 1) * (labn + 1), sizeof(int));
     int nf = 0;
+    ORC_PHASE(ORC_PH_T2);
     int p = 0 > shifta ? 0 : shifta, q = 0 > shiftb ? 0 : shiftb;
     const double *Fa = F->a + p * Finc3 + sha * Finc2;
     const double *Fb = F->a + q * Finc3 + shb * Finc2;
@@ -658,6 +667,7 @@ static double *calc_gamma(orc_handle *h, const arr_t *F, const arr_t *UT, const 
       return NULL;
     }
     free(fl);
+    ORC_PHASE(ORC_PH_ASM2);
     for (int alpha = 0; alpha <= la; alpha++) {
       int parity = (alpha + l) % 2;
       int ll1 = l - alpha;
@@ -863,7 +873,9 @@ int orc_calculate(orc_handle *h, orc_callback cb, void *args) {
     arr_t UT = tab_ecp(U, h->grid2, h->maxLambda, h->acc2);
     arr_t FT = calc_f(h, rC, &sg, &ferr);
     if (ferr) result = 2;
+    ORC_PHASE(ORC_PH_UTAB);
     for (int s = 0; s < h->grid1->order; s++) U_L[s] = orc_eval_ecp(U, U->L, h->grid1->x[s]);
+    ORC_PHASE(ORC_PH_OTHER);
     int sOff1 = 0, pOff1 = 0;
     for (int A = 0; A < nAt && !result; A++) {
       const double *rA = h->geom + 3 * A;
@@ -873,8 +885,10 @@ int orc_calculate(orc_handle *h, orc_callback cb, void *args) {
         double x = rC[0] - rA[0], y = rC[1] - rA[1], z = rC[2] - rA[2];
         dAC = sqrt(x * x + y * y + z * z);
       }
+      ORC_PHASE(ORC_PH_ANG);
       arr_t uspA = usp_table(h->maxLAtom[A] + d, rAC);
       arr_t omA = eval_angular(h, &omega, h->maxLAtom[A] + d, rAC);
+      ORC_PHASE(ORC_PH_OTHER);
       int sOff2 = sOff1, pOff2 = pOff1;
       for (int B = A; B < nAt && !result; B++) {
         const double *rB = h->geom + 3 * B;
@@ -884,8 +898,10 @@ int orc_calculate(orc_handle *h, orc_callback cb, void *args) {
           double x = rC[0] - rB[0], y = rC[1] - rB[1], z = rC[2] - rB[2];
           dBC = sqrt(x * x + y * y + z * z);
         }
+        ORC_PHASE(ORC_PH_ANG);
         arr_t uspB = usp_table(h->maxLAtom[B] + d, rBC);
         arr_t omB = eval_angular(h, &omega, h->maxLAtom[B] + d, rBC);
+        ORC_PHASE(ORC_PH_OTHER);
         sIdx1 = sOff1;
         pIdx1 = pOff1;
         for (int s1 = 0; s1 < h->shells[A] && !result; s1++) {
@@ -1014,3 +1030,18 @@ int orc_getIntegrals(int nrAtoms, const double *geometry, const int *shellsECP, 
   (void)rc; /* the reference ignores the integration status, getIntegrals.c:88 */
   return 0;
 }
+
+#ifdef ORC_OPCOUNT
+/* Op-counting build: FP64 operations executed per phase (FMA = 2, every
+   add/sub/mul/div and every libm call = 1), reset and read by the tests. */
+long long orc_ops[16];
+int orc_phase;
+void orc_ops_reset(void) {
+  memset(orc_ops, 0, sizeof(orc_ops));
+  orc_phase = ORC_PH_OTHER;
+}
+int orc_ops_get(long long *out, int n) {
+  for (int i = 0; i < ORC_PH_N && i < n; i++) out[i] = orc_ops[i];
+  return ORC_PH_N;
+}
+#endif

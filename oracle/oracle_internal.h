@@ -40,6 +40,19 @@ typedef struct {
   orc_gauss *g;
 } orc_ecp;
 
+/* Op-counting build (liboracle_ops.so, compiled as C++ with a counting
+   double): the phase the arithmetic is attributed to.  A no-op otherwise. */
+enum {
+  ORC_PH_OTHER = 0, ORC_PH_UTAB, ORC_PH_FTAB, ORC_PH_ANG, ORC_PH_T1, ORC_PH_FB1, ORC_PH_ASM1,
+  ORC_PH_T2, ORC_PH_FB2, ORC_PH_ASM2, ORC_PH_SHIFT, ORC_PH_N
+};
+#ifdef ORC_OPCOUNT
+extern int orc_phase;
+#define ORC_PHASE(p) (orc_phase = (p))
+#else
+#define ORC_PHASE(p) ((void)0)
+#endif
+
 /* tables */
 double *orc_fac_table(int n, int step);
 int *orc_cart_order(int am);

@@ -65,7 +65,7 @@ int orc_tm_offset(int l) {
 }
 
 static double orc_binom(int n, int k, const double *fac) {
-  return (k >= 0 && k <= n) ? fac[n] / (fac[n - k] * fac[k]) : 0.0;
+  return (k >= 0 && k <= n) ? fac[n] / (fac[n - k] * fac[k]) : (double)0.0;
 }
 
 /* Real-solid-harmonic coefficients of the Cartesian monomials, Schlegel &
@@ -293,7 +293,7 @@ void orc_map_kk(orc_grid *g) {
 void orc_map_fm06(orc_grid *g, double zeta_P, double P) {
   double sigma = 1.0 / sqrt(zeta_P);
   double t = P - 7.0 * sigma;
-  double rmin = (t > 0.0) ? t : 0.0;
+  double rmin = (t > 0.0) ? t : (double)0.0;
   double rmax = P + 9.0 * sigma;
   double i1 = 0.5 * (rmax - rmin);
   double i2 = 0.5 * (rmax + rmin);
@@ -584,7 +584,7 @@ void orc_sph_coords(const double *xyz, double *out) {
   const double acc = 1.0E-14;
   const double x = xyz[0], y = xyz[1], z = xyz[2];
   double r = sqrt(x * x + y * y + z * z), theta, phi;
-  theta = (r < acc) ? 0.0 : acos(z / r);
+  theta = (r < acc) ? (double)0.0 : acos(z / r);
   if (fabs(x) < acc) {
     if (fabs(y) < acc) phi = 0.0;
     else if (y < 0.0) phi = 1.5 * M_PI;

@@ -1,7 +1,8 @@
 """Launch parameters and the autotuner change how the type-1 small-grid
 integration is split between kernels (lane-serial head, wave-cooperative or
-lane-serial tail), never its arithmetic: every setting must give the same
-matrix bit for bit."""
+lane-serial tail), whether the type-1 fallback starts beside the tail
+(speculatively) or after it, and how the other stages are launched, never
+their arithmetic: every setting must give the same matrix bit for bit."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,8 @@ SETTINGS = [
     {"fuse": 0, "t1_coop_max": 0, "t1_head_serial": 6, "t1_tail_grid": 3},
     {"fuse": 1, "fb2_split": 1},                                  # one wave per (triplet, pair)
     {"fuse": 0, "fb2_split": 4},
+    {"fuse": 1, "t1_spec": 0},                                    # type-1 fallback after the tail
+    {"fuse": 1, "t1_spec": 1, "t1_head_coop": 2},                 # ... beside it, from level 2 on
 ]
 
 
