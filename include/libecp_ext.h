@@ -117,6 +117,20 @@ int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I);
    device (no host copies).  *ms (ms may be NULL): device time of the
    contraction kernels.  0, or -1. */
 int libecp_batch_contract(libecp_batch *b, const double *P, double *out, int *codes, int device, double *ms);
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  libecp_batch_contract with device buffers (P, out on
+   the batch's device; codes on the host, may be NULL) that calls no
+   hipMalloc, hipFree or other allocating routine: the index tables are
+   built when the batch is created, and the partial sums, codes and index
+   tables live in a static device array of 32 MiB per GPU, held until the
+   process ends.  Jobs needing more (about 4 million triplets for the
+   energy, 1.3 million shifted triplets for the gradient) are rejected with
+   a message.  Every call on a GPU shares that GPU's array: calls that
+   overlap on one GPU (other threads, other batches) race and must be
+   serialised by the caller.  Derivative orders 0 and 1.  Results equal
+   libecp_batch_contract's bit for bit.  0, or -1 (libecp_last_error()). */
+int libecp_batch_contract_noalloc(libecp_batch *b, const double *P, double *out, int *codes, double *ms);
 /* Nominal (pre-screening) and evaluated triplet counts of the batch. */
 long long libecp_batch_triplets(libecp_batch *b, long long *evaluated);
 /* Doubles of the evaluated blocks (type 1 and type 2) the batch holds on

@@ -537,6 +537,8 @@ struct libecp_batch {
   double *blocks;
   unsigned char *status;
   long long nominal;
+  char *con_img;        /* contraction index tables of the packed density layout */
+  size_t con_img_bytes;
 };
 
 void libecp_batch_free(libecp_batch *b) {
@@ -551,6 +553,7 @@ void libecp_batch_free(libecp_batch *b) {
   free(b->nat);
   free(b->blocks);
   free(b->status);
+  free(b->con_img);
   free(b);
 }
 
@@ -620,6 +623,30 @@ libecp_batch *libecp_batch_create_n(const libecp_molecule *mols, int nmol, int n
     return NULL;
   }
   ecp_job_ao_offsets(&b->job, b->aoff);
+  if (n <= 1) {
+    /* the index tables of libecp_batch_contract_noalloc (densities packed
+       by molecule), built here so that the call itself allocates nothing */
+    long long *poff = malloc(sizeof(long long) * (size_t)nmol);
+    int *prow = malloc(sizeof(int) * (size_t)nmol);
+    size_t nP = 0;
+    if (poff && prow) {
+      for (int m = 0; m < nmol; m++) {
+        poff[m] = (long long)nP;
+        prow[m] = b->nao[m];
+        nP += (size_t)b->nao[m] * b->nao[m];
+      }
+      b->con_img_bytes = ecp_dev_contract_image(&b->job, poff, prow, NULL, 0);
+      b->con_img = malloc(b->con_img_bytes ? b->con_img_bytes : 1);
+      if (b->con_img) ecp_dev_contract_image(&b->job, poff, prow, b->con_img, b->con_img_bytes);
+    }
+    free(poff);
+    free(prow);
+    if (!b->con_img) {
+      set_last_error("out of host memory for the batch's contraction tables");
+      libecp_batch_free(b);
+      return NULL;
+    }
+  }
   b->dev = ecp_dev_create(&b->st, err, sizeof(err));
   if (!b->dev || ecp_dev_prepare(b->dev, &b->job, err, sizeof(err))) {
     set_last_error(err);
@@ -656,6 +683,24 @@ int libecp_batch_matrix(libecp_batch *b, int mol, int rowdim, double *I) {
     return -1;
   }
   return ecp_job_scatter(&b->job, mol, b->aoff, b->blocks, b->status, rowdim, 0, I);
+}
+
+int libecp_batch_contract_noalloc(libecp_batch *b, const double *P, double *out, int *codes, double *ms) {
+  char err[512] = "";
+  if (!b || !P || !out) {
+    set_last_error("NULL argument");
+    return -1;
+  }
+  if (b->n > 1 || !b->con_img) {
+    set_last_error("density contraction supports derivative orders 0 and 1");
+    return -1;
+  }
+  if (ecp_dev_contract_noalloc(b->dev, &b->job, b->con_img, b->con_img_bytes, P, out, codes, ms, err,
+                               sizeof(err))) {
+    set_last_error(err);
+    return -1;
+  }
+  return 0;
 }
 
 int libecp_batch_contract(libecp_batch *b, const double *P, double *out, int *codes, int device, double *ms) {

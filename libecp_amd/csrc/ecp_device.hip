@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ecp_host.h"
@@ -51,6 +52,44 @@
       return -1;                                                                   \
     }                                                                              \
   } while (0)
+
+// Device scratch of the no-allocation entry point (ecp_dev_contract_noalloc):
+// one array per GPU, held until the process ends.  Host code never uses the
+// name `scratch` as an address: set-up looks up each device's copy with
+// hipGetSymbolAddress, and every use goes through that pointer.
+constexpr size_t kScratchBytes = (size_t)32 << 20;  // 32 MiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices];
+
+// The scratch address on the current device, looked up once per device.
+// 0, or -1 (message in err).
+int scratch_setup(void **ptr, char *err, size_t errlen) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kScratchDevices) {
+    snprintf(err, errlen, "no-allocation scratch: no usable current device (%s)", hipGetErrorString(e));
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      snprintf(err, errlen, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return -1;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
 
 namespace {
 
@@ -2328,6 +2367,7 @@ struct ecp_dev {
   std::vector<char> con_idx;  // index tables last uploaded to con_mem at con_idx_at
   size_t con_idx_at = 0;
   double *fbf = nullptr;      // the profile's fallback FLOP sums
+  void *scratch = nullptr;    // this device's no-allocation scratch (scratch_setup)
 };
 
 namespace {
@@ -2436,6 +2476,12 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
       if ((e = hipEventCreate(&d->span[k][i])) != hipSuccess) return fail("hipEventCreate", e);
   for (int i = 0; i < 8; i++)
     if ((e = hipEventCreate(&d->ev[i])) != hipSuccess) return fail("hipEventCreate", e);
+  for (int i = 0; i < 2; i++)
+    if ((e = hipEventCreate(&d->con_ev[i])) != hipSuccess) return fail("hipEventCreate", e);
+  if (scratch_setup(&d->scratch, err, errlen)) {
+    ecp_dev_destroy(d);
+    return nullptr;
+  }
   build_templates(d);
   const int so = ECP_SMALL_ORDER, lo = st->lorder;
   const size_t nbess = (size_t)(ECP_BESSEL_N + 1) * ECP_BESSEL_STRIDE;
@@ -3696,6 +3742,103 @@ __global__ __launch_bounds__(kConThreads) void k_con_reduce(ConArgs A) {
 }
 }  // namespace
 
+namespace {
+
+// Index tables of a contraction (device layout, one block of `bytes`): the
+// density offsets and row strides per molecule, the AO offset of every
+// shell, the triplet range of every molecule, and the output units (one
+// per molecule for the energy, one per atom for the gradient: molecule,
+// global atom, output offset).  img == nullptr: sizes only.
+struct ConImage {
+  size_t poff = 0, aoff = 0, prow = 0, tmb = 0, umol = 0, uatom = 0, uout = 0, bytes = 0;
+  int nunits = 0;
+  size_t nout = 0;
+};
+
+ConImage con_image(const ecp_job *job, const long long *poff, const int *prow, char *img) {
+  ConImage c;
+  const int nmol = job->nmol, nd = job->deriv == 0 ? 1 : 3;
+  c.nunits = nd == 1 ? nmol : job->mol_atom0[nmol];
+  c.nout = (size_t)c.nunits * nd;
+  size_t q = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = q;
+    q += (bytes + 7) & ~(size_t)7;
+    return o;
+  };
+  c.poff = take(8 * (size_t)nmol);
+  c.aoff = take(4 * ((size_t)job->nsh + 1));
+  c.prow = take(4 * (size_t)nmol);
+  c.tmb = take(4 * ((size_t)nmol + 1));
+  c.umol = take(4 * (size_t)c.nunits);
+  c.uatom = take(4 * (size_t)c.nunits);
+  c.uout = take(4 * (size_t)c.nunits);
+  c.bytes = q;
+  if (!img) return c;
+  memcpy(img + c.poff, poff, 8 * (size_t)nmol);
+  ecp_job_ao_offsets(job, (int *)(img + c.aoff));
+  memcpy(img + c.prow, prow, 4 * (size_t)nmol);
+  memcpy(img + c.tmb, job->trip_mol_begin, 4 * ((size_t)nmol + 1));
+  int *um = (int *)(img + c.umol), *ua = (int *)(img + c.uatom), *uo = (int *)(img + c.uout);
+  int u = 0;
+  for (int m = 0; m < nmol; m++) {
+    if (nd == 1) {
+      um[u] = m;
+      ua[u] = -1;
+      uo[u] = u;
+      u++;
+      continue;
+    }
+    for (int g = job->mol_atom0[m]; g < job->mol_atom0[m + 1]; g++, u++) {
+      um[u] = m;
+      ua[u] = g;
+      uo[u] = 3 * u;
+    }
+  }
+  return c;
+}
+
+// Launches the contraction over a prepared image at base + o_idx, partial
+// sums at part, codes at codes_dev; P and out are device pointers.
+int con_launch(ecp_dev *d, const ecp_job *job, const ConImage &c, char *idx, const double *P, double *part,
+               double *out, int *codes_dev, char *err, size_t errlen) {
+  const Prepared &Pp = d->P;
+  ConArgs A{};
+  A.trip = Pp.J.trip;
+  A.shells = Pp.J.shells;
+  A.cent = Pp.J.cent;
+  A.blocks = Pp.blk;
+  A.stat = Pp.stat;
+  A.aoff = (const int *)(idx + c.aoff);
+  A.P = P;
+  A.poff = (const long long *)(idx + c.poff);
+  A.prow = (const int *)(idx + c.prow);
+  A.tmb = (const int *)(idx + c.tmb);
+  A.umol = (const int *)(idx + c.umol);
+  A.uatom = (const int *)(idx + c.uatom);
+  A.uout = (const int *)(idx + c.uout);
+  A.part = part;
+  A.out = out;
+  A.codes = codes_dev;
+  A.ntrip = Pp.nt;
+  A.nd = job->deriv == 0 ? 1 : 3;
+  A.nunits = c.nunits;
+  A.nmol = job->nmol;
+  ECP_CHECK(hipEventRecord(d->con_ev[0], d->stream));
+  if (Pp.nt > 0) {
+    k_con_part<<<std::min(Pp.nt, 65536), 64, 0, d->stream>>>(d->T, A);
+    ECP_CHECK(hipGetLastError());
+  }
+  if (c.nunits > 0) {
+    k_con_reduce<<<c.nunits, kConThreads, 0, d->stream>>>(A);
+    ECP_CHECK(hipGetLastError());
+  }
+  ECP_CHECK(hipEventRecord(d->con_ev[1], d->stream));
+  return 0;
+}
+
+}  // namespace
+
 extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P, size_t nP, int P_on_device,
                                 const long long *poff, const int *prow, double *out, int out_on_device,
                                 int *codes, double *ms, char *err, size_t errlen) {
@@ -3709,36 +3852,16 @@ extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P,
     return -1;
   }
   const int nmol = job->nmol, nd = job->deriv == 0 ? 1 : 3;
-  // output units: one per molecule (energy) or per atom (gradient)
-  std::vector<int> umol, uatom, uout, aoff((size_t)job->nsh + 1);
-  ecp_job_ao_offsets(job, aoff.data());
-  size_t nout = 0;
-  if (nd == 1) {
-    for (int m = 0; m < nmol; m++) {
-      umol.push_back(m);
-      uatom.push_back(-1);
-      uout.push_back((int)nout++);
-    }
-  } else {
-    for (int m = 0; m < nmol; m++) {
-      for (int g = job->mol_atom0[m]; g < job->mol_atom0[m + 1]; g++) {
-        umol.push_back(m);
-        uatom.push_back(g);
-        uout.push_back((int)nout);
-        nout += 3;
-      }
-    }
-  }
-  const int nunits = (int)umol.size();
+  const ConImage ci = con_image(job, poff, prow, nullptr);
+  std::vector<char> h(ci.bytes);
+  con_image(job, poff, prow, h.data());
   // device layout
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t o = 0;
   const size_t o_part = o; o += al(8 * (size_t)std::max(1, Pp.nt) * nd);
-  const size_t o_out = o; o += al(8 * std::max<size_t>(1, nout));
+  const size_t o_out = o; o += al(8 * std::max<size_t>(1, ci.nout));
   const size_t o_codes = o; o += al(4 * (size_t)nmol);
-  const size_t o_idx = o;
-  const size_t n_idx = 8 * (size_t)nmol + 4 * ((size_t)job->nsh + 2 * (size_t)nmol + 1 + 3 * (size_t)nunits);
-  o += al(n_idx);
+  const size_t o_idx = o; o += al(ci.bytes);
   const size_t o_P = o; o += P_on_device ? 0 : al(8 * nP);
   if (o > d->con_cap) {
     if (d->con_mem) ECP_CHECK(hipFree(d->con_mem));
@@ -3748,67 +3871,82 @@ extern "C" int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P,
     d->con_cap = o;
     d->con_idx.clear();
   }
-  if (!d->con_ev[0]) {
-    ECP_CHECK(hipEventCreate(&d->con_ev[0]));
-    ECP_CHECK(hipEventCreate(&d->con_ev[1]));
-  }
-  std::vector<char> h(n_idx);
-  size_t q = 0;
-  auto put = [&](const void *src, size_t bytes) {
-    memcpy(h.data() + q, src, bytes);
-    q += bytes;
-    return o_idx + q - bytes;
-  };
-  ConArgs A{};
   char *b = d->con_mem;
-  const size_t i_poff = put(poff, 8 * (size_t)nmol);
-  const size_t i_aoff = put(aoff.data(), 4 * (size_t)job->nsh);
-  const size_t i_prow = put(prow, 4 * (size_t)nmol);
-  const size_t i_tmb = put(job->trip_mol_begin, 4 * ((size_t)nmol + 1));
-  const size_t i_umol = put(umol.data(), 4 * (size_t)nunits);
-  const size_t i_uatom = put(uatom.data(), 4 * (size_t)nunits);
-  const size_t i_uout = put(uout.data(), 4 * (size_t)nunits);
   // the index tables change with the job or the density layout only
-  if (d->con_idx_at != o_idx || d->con_idx.size() != q || memcmp(d->con_idx.data(), h.data(), q) != 0) {
-    ECP_CHECK(hipMemcpyAsync(b + o_idx, h.data(), q, hipMemcpyHostToDevice, d->stream));
+  if (d->con_idx_at != o_idx || d->con_idx != h) {
+    ECP_CHECK(hipMemcpyAsync(b + o_idx, h.data(), h.size(), hipMemcpyHostToDevice, d->stream));
     ECP_CHECK(hipStreamSynchronize(d->stream));
-    d->con_idx.assign(h.begin(), h.begin() + q);
+    d->con_idx = h;
     d->con_idx_at = o_idx;
   }
   if (!P_on_device && nP) ECP_CHECK(hipMemcpyAsync(b + o_P, P, 8 * nP, hipMemcpyHostToDevice, d->stream));
-  A.trip = Pp.J.trip;
-  A.shells = Pp.J.shells;
-  A.cent = Pp.J.cent;
-  A.blocks = Pp.blk;
-  A.stat = Pp.stat;
-  A.aoff = (const int *)(b + i_aoff);
-  A.P = P_on_device ? P : (const double *)(b + o_P);
-  A.poff = (const long long *)(b + i_poff);
-  A.prow = (const int *)(b + i_prow);
-  A.tmb = (const int *)(b + i_tmb);
-  A.umol = (const int *)(b + i_umol);
-  A.uatom = (const int *)(b + i_uatom);
-  A.uout = (const int *)(b + i_uout);
-  A.part = (double *)(b + o_part);
-  A.out = out_on_device ? out : (double *)(b + o_out);
-  A.codes = (int *)(b + o_codes);
-  A.ntrip = Pp.nt;
-  A.nd = nd;
-  A.nunits = nunits;
-  A.nmol = nmol;
-  ECP_CHECK(hipEventRecord(d->con_ev[0], d->stream));
-  if (Pp.nt > 0) {
-    k_con_part<<<std::min(Pp.nt, 65536), 64, 0, d->stream>>>(d->T, A);
-    ECP_CHECK(hipGetLastError());
-  }
-  if (nunits > 0) {
-    k_con_reduce<<<nunits, kConThreads, 0, d->stream>>>(A);
-    ECP_CHECK(hipGetLastError());
-  }
-  ECP_CHECK(hipEventRecord(d->con_ev[1], d->stream));
-  if (!out_on_device && nout)
-    ECP_CHECK(hipMemcpyAsync(out, b + o_out, 8 * nout, hipMemcpyDeviceToHost, d->stream));
+  double *outp = out_on_device ? out : (double *)(b + o_out);
+  if (con_launch(d, job, ci, b + o_idx, P_on_device ? P : (const double *)(b + o_P), (double *)(b + o_part), outp,
+                 (int *)(b + o_codes), err, errlen))
+    return -1;
+  if (!out_on_device && ci.nout)
+    ECP_CHECK(hipMemcpyAsync(out, b + o_out, 8 * ci.nout, hipMemcpyDeviceToHost, d->stream));
   ECP_CHECK(hipMemcpyAsync(codes, b + o_codes, 4 * (size_t)nmol, hipMemcpyDeviceToHost, d->stream));
+  ECP_CHECK(hipStreamSynchronize(d->stream));
+  if (ms) {
+    float f = 0.f;
+    ECP_CHECK(hipEventElapsedTime(&f, d->con_ev[0], d->con_ev[1]));
+    *ms = f;
+  }
+  return 0;
+}
+
+extern "C" size_t ecp_dev_contract_image(const ecp_job *job, const long long *poff, const int *prow, char *img,
+                                         size_t cap) {
+  const ConImage c = con_image(job, poff, prow, nullptr);
+  if (img && cap >= c.bytes) con_image(job, poff, prow, img);
+  return c.bytes;
+}
+
+extern "C" int ecp_dev_contract_noalloc(ecp_dev *d, const ecp_job *job, const char *img, size_t img_bytes,
+                                        const double *P, double *out, int *codes, double *ms, char *err,
+                                        size_t errlen) {
+  const Prepared &Pp = d->P;
+  if (!d->prepared) {
+    snprintf(err, errlen, "no evaluated job to contract");
+    return -1;
+  }
+  if (job->deriv > 1) {
+    snprintf(err, errlen, "density contraction supports derivative orders 0 and 1");
+    return -1;
+  }
+  if (!d->scratch) {
+    snprintf(err, errlen, "the no-allocation scratch of this device is not set up");
+    return -1;
+  }
+  const int nmol = job->nmol, nd = job->deriv == 0 ? 1 : 3;
+  // the image's sizes, from the same layout function that wrote it (no
+  // allocation: sizes only)
+  const ConImage ci = con_image(job, nullptr, nullptr, nullptr);
+  if (ci.bytes != img_bytes) {
+    snprintf(err, errlen, "contraction index image of %zu bytes does not match the job (%zu)", img_bytes, ci.bytes);
+    return -1;
+  }
+  // scratch layout: partial sums, codes, index tables
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t o = 0;
+  const size_t o_part = o; o += al(8 * (size_t)std::max(1, Pp.nt) * nd);
+  const size_t o_codes = o; o += al(4 * (size_t)nmol);
+  const size_t o_idx = o; o += al(ci.bytes);
+  if (o > kScratchBytes) {
+    snprintf(err, errlen,
+             "the contraction needs %zu bytes of scratch (%d triplets, %d molecules); the no-allocation "
+             "entry point holds %zu",
+             o, Pp.nt, nmol, kScratchBytes);
+    return -1;
+  }
+  char *b = (char *)d->scratch;
+  // nothing in the scratch has to start at zero: the kernels write every
+  // partial sum, output and code they read
+  ECP_CHECK(hipMemcpyAsync(b + o_idx, img, ci.bytes, hipMemcpyHostToDevice, d->stream));
+  if (con_launch(d, job, ci, b + o_idx, P, (double *)(b + o_part), out, (int *)(b + o_codes), err, errlen))
+    return -1;
+  if (codes) ECP_CHECK(hipMemcpyAsync(codes, b + o_codes, 4 * (size_t)nmol, hipMemcpyDeviceToHost, d->stream));
   ECP_CHECK(hipStreamSynchronize(d->stream));
   if (ms) {
     float f = 0.f;

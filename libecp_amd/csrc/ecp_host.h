@@ -123,6 +123,17 @@ int ecp_dev_fetch(ecp_dev *d, double *blocks, unsigned char *status, char *err, 
 int ecp_dev_contract(ecp_dev *d, const ecp_job *job, const double *P, size_t nP, int P_on_device,
                      const long long *poff, const int *prow, double *out, int out_on_device, int *codes,
                      double *ms, char *err, size_t errlen);
+/* The contraction's index tables for densities at poff / prow (the layout
+   ecp_dev_contract uploads): returns their size in bytes and writes them to
+   img when cap is large enough. */
+size_t ecp_dev_contract_image(const ecp_job *job, const long long *poff, const int *prow, char *img, size_t cap);
+/* ecp_dev_contract with device P and out and no allocation: the index
+   tables (img, from ecp_dev_contract_image), partial sums and codes go to
+   the device's static scratch (32 MiB; larger jobs are rejected with a
+   message).  Calls that overlap on one device share the scratch and race.
+   codes (host, may be NULL).  0, or -1 (message in err). */
+int ecp_dev_contract_noalloc(ecp_dev *d, const ecp_job *job, const char *img, size_t img_bytes, const double *P,
+                             double *out, int *codes, double *ms, char *err, size_t errlen);
 /* One eager evaluation timed per kernel launch (HIP events on the launch's
    own stream).  Per kernel family (tables, type1, type2, fallback1,
    fallback2, assemble): launches, summed duration in ms, algorithmic FLOPs
