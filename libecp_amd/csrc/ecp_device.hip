@@ -1172,37 +1172,8 @@ __device__ __forceinline__ int t2_entry(const DevTables &T, const DevJob &J, con
   for (int lv = 2; lv < lv_end; lv++) {
     const int j = T.s93.lvl_j[lv], nn = T.s93.lvl_n[lv];
     int cnt = 0;
-    // four point pairs at a time: their loads are issued together (a lane
-    // that runs every level is otherwise one long chain of dependent
-    // reads), both points evaluated whatever the window (the tables hold
-    // every point), the sums T = (0 + f(left)) + f(right) added in order.
-    // The active lanes walk the levels in step, so the pair index is
-    // uniform and the schedule reads are scalar.
-    const int kb = T.s93.lvl_begin[lv], ke = T.s93.lvl_begin[lv + 1];
-    int k = kb;
-    for (; k + 4 <= ke; k += 4) {
-      const int ku = __builtin_amdgcn_readfirstlane(k);
-      double t[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int li = T.p93l[ku + i], ri = T.p93r[ku + i];
-        const double fl = f(li), fr = f(ri);
-        t[i] = 0.0;
-        if (li >= gs) {
-          cnt++;
-          t[i] += fl;
-        }
-        if (ri <= ge) {
-          cnt++;
-          t[i] += fr;
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 4; i++) I += t[i];
-    }
-    for (; k < ke; k++) {
-      const int ku = __builtin_amdgcn_readfirstlane(k);
-      const int li = T.p93l[ku], ri = T.p93r[ku];
+    for (int k = T.s93.lvl_begin[lv]; k < T.s93.lvl_begin[lv + 1]; k++) {
+      const int li = T.p93l[k], ri = T.p93r[k];
       double t = 0.0;
       if (li >= gs) {
         cnt++;
@@ -1610,19 +1581,6 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     }
     __syncthreads();
   }
-}
-
-template <int LMB>
-__global__ __launch_bounds__(64) void k_fb2(DevTables T, DevJob J, const T2Group *__restrict__ groups,
-                                            const int *__restrict__ tmpl, const unsigned char *__restrict__ tfail,
-                                            const int *__restrict__ fbq, const int *__restrict__ fbcount,
-                                            double *__restrict__ fbT, unsigned char *__restrict__ status,
-                                            const int *__restrict__ gcount, const int *__restrict__ flist,
-                                            const int *__restrict__ tgfirst, int nsplit) {
-  __shared__ double s_r[64], s_w[64], s_cu[ECP_MAX_L * 64], s_ex[64];
-  __shared__ double s_ka[(LMB + 1) * kKS], s_kb[(LMB + 1) * kKS];
-  fb2_body<LMB>(blockIdx.x, gridDim.x, T, J, groups, tmpl, tfail, fbq, fbcount, fbT, status, gcount, flist, tgfirst,
-                nsplit, s_r, s_w, s_cu, s_ex, s_ka, s_kb);
 }
 
 // ---------------------------------------------------------------------------
@@ -2748,18 +2706,6 @@ int fb2_split(const ecp_dev *d, bool fused) {
   return items <= kFb2SplitMaxItems ? 3 : 1;
 }
 
-template <int LMB>
-void launch_fb2(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, const T2Group *g, const int *tm,
-                const unsigned char *tf, const int *fbq, const int *fbc, double *fbT, unsigned char *st,
-                const int *gc, const int *fl, const int *tgf, long maxItems) {
-  if (maxItems > 0) {
-    Span sp(d, s, FAM_FB2);
-    const int nsplit = fb2_split(d, false);
-    const long blocks = std::min<long>(maxItems * nsplit, 8192);
-    k_fb2<LMB><<<(int)blocks, 64, 0, s>>>(T, J, g, tm, tf, fbq, fbc, fbT, st, gc, fl, tgf, nsplit);
-  }
-}
-
 // The kernel sequence of one evaluation, on resident inputs, one launch per
 // class.  Tables run on the main stream.  The type-1 classes run on three
 // forked streams, each class as small grid -> large-grid fallback ->
@@ -2794,6 +2740,24 @@ void launch_ang(ecp_dev *d, hipStream_t s) {
 // fuse = 2 (automatic): fused launches for jobs of up to this many type-1
 // primitive pairs (latency-bound: per-class launches hold few waves each)
 constexpr int kFuseAutoMaxUnits = 150000;
+
+// workgroup ranges of the classes with work; blocks(c) = workgroups of class c
+template <typename F>
+ClassMap class_map(int ncls, const int *count, F blocks) {
+  ClassMap cm{};
+  int b = 0;
+  for (int c = 0; c < ncls; c++) {
+    if (count[c] <= 0) continue;
+    const int nb = blocks(c);
+    if (nb <= 0) continue;
+    cm.cls[cm.n] = c;
+    cm.bbeg[cm.n] = b;
+    b += nb;
+    cm.n++;
+  }
+  cm.bbeg[cm.n] = b;
+  return cm;
+}
 
 // one type-1 class of the per-class sequence: small grid, fallback, assembly
 template <int c>
@@ -2856,11 +2820,20 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       launch_asm2(d, w2, 1, false);
     }
     (void)hipStreamWaitEvent(s, d->t2done, 0);
-#define FB2(c)                                                                                                  \
-  launch_fb2<c>(d, s, T, J, P.grp, d->tmpl_dev, P.tf, P.fbq2 + (size_t)(c)*P.nt, P.fbc + kFb2Cnt + (c), P.fbT,  \
-                P.stat, P.gflag, P.flist, P.tgf, (long)P.tcount2[c] * P.maxPairs);
-    FB2(0) FB2(1) FB2(2) FB2(3) FB2(4) FB2(5) FB2(6) FB2(7) FB2(8) FB2(9) FB2(10) FB2(11)
-#undef FB2
+    // every fallback class in one persistent grid: the classes' items run
+    // side by side instead of one class after another on this stream
+    {
+      const int nsplit = fb2_split(d, false);
+      T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.nt, P.gflag, P.flist, P.tgf, nsplit};
+      const ClassMap cm2 = class_map(kMaxLMB + 1, P.tcount2, [&](int c) {
+        return (int)std::min<long>((long)P.tcount2[c] * P.maxPairs * nsplit, 8192);
+      });
+      if (cm2.n > 0) {
+        Span sp(d, s, FAM_FB2);
+        if (P.max_cls2 > 7) k_fb2_fused<11><<<cm2.bbeg[cm2.n], 64, 0, s>>>(T, J, S2, cm2);
+        else k_fb2_fused<7><<<cm2.bbeg[cm2.n], 64, 0, s>>>(T, J, S2, cm2);
+      }
+    }
     {
       Span sp(d, s, FAM_ASM);
       launch_asm2(d, s, 2, false);
@@ -2892,23 +2865,6 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   }
 }
 
-// workgroup ranges of the classes with work; blocks(c) = workgroups of class c
-template <typename F>
-ClassMap class_map(int ncls, const int *count, F blocks) {
-  ClassMap cm{};
-  int b = 0;
-  for (int c = 0; c < ncls; c++) {
-    if (count[c] <= 0) continue;
-    const int nb = blocks(c);
-    if (nb <= 0) continue;
-    cm.cls[cm.n] = c;
-    cm.bbeg[cm.n] = b;
-    b += nb;
-    cm.n++;
-  }
-  cm.bbeg[cm.n] = b;
-  return cm;
-}
 
 // The fused sequence: tables on the main stream, then two chains.  Type 1:
 // head (all classes) -> tail -> [assembly of triplets without large-grid
