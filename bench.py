@@ -18,7 +18,7 @@ triplets/s):
                reference per call
   high_l       the 20-atom g-shell configuration in one resident job, and
                its distance from the oracle element by element
-  batched      256 molecules in one resident job (autotuned launch
+  batched      256 molecules in one resident job (default launch
                parameters), fresh-job end to end, and the same molecules
                through getIntegrals one by one
   serving      a Poisson request stream against the request server
@@ -525,7 +525,7 @@ def leg_batched(args, ctx):
     bb.execute(1)
     t_first = time.perf_counter() - t
     nom_b, ev_b = bb.triplets()
-    tune_b = bb.autotune(2) if not args.no_autotune else "defaults"
+    tune_b = bb.autotune(2) if args.autotune_batched else "defaults"
     params = {k: bb.get_param(k) for k in ("fuse", "asm2_cache", "t1_head_coop", "t1_head_serial", "t1_coop_max",
                                            "t1_tail_grid", "t2_head", "t2_tail_grid", "fb2_split")}
     bb.execute(2)
@@ -629,7 +629,10 @@ def main():
     ap.add_argument("--only", choices=LEGS, help="run one leg and print its result (profiling)")
     ap.add_argument("--autotune", action="store_true",
                     help="run the launch-parameter autotuner on the medium cluster before timing")
-    ap.add_argument("--no-autotune", action="store_true", help="the uneven batch with default launch parameters")
+    ap.add_argument("--no-autotune", action="store_true", help="accepted for compatibility: the uneven batch runs "
+                    "with default launch parameters unless --autotune-batched is given")
+    ap.add_argument("--autotune-batched", action="store_true",
+                    help="run the launch-parameter autotuner on the uneven batch before timing")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=float, default=32.0)

@@ -156,7 +156,8 @@ const char *libecp_kernel_family(int i);
 int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples);
 /* Launch parameters of the batch's device pipeline, by name:
    "t1_head_coop" / "t1_head_serial" (PS93 levels the type-1 head kernel runs
-   before the wave-cooperative / lane-serial tail), "t1_coop_max" (largest
+   before the wave-cooperative / lane-serial tail; 8 and 8 by default),
+   "t1_coop_max" (largest
    type-1 class, in primitive pairs, that takes the cooperative tail),
    "t1_tail_grid" (most workgroups of a tail launch), "fuse" (1: one launch
    per stage covering every class, with assembly overlapping the large-grid

@@ -24,9 +24,9 @@
 #define ECP_LARGE_MAX 2047
 #define ECP_LARGE_LEVELS_MAX 11
 
-/* Limits of this build.  Basis l <= 5, ECP local label L <= 7, derivative
-   order 0.  The Bessel table carries rows 0..ECP_BESSEL_ROWS-1. */
-#define ECP_MAX_LBS 5
+/* Limits of this build.  Basis l plus derivative order <= 6 (type-1
+   classes la + lb <= 12, Bessel orders <= 12), ECP local label L <= 7.  The Bessel table carries rows 0..ECP_BESSEL_ROWS-1. */
+#define ECP_MAX_LBS 6
 #define ECP_MAX_L 7
 #define ECP_MAX_LAB (2 * ECP_MAX_LBS)
 #define ECP_BESSEL_ROWS 24

@@ -426,7 +426,7 @@ __device__ void ftab_item(const DevTables &T, const DevJob &J, int2 it) {
   case k:     \
     ftab_slab<k>(T, J, C, sl, run, W); \
     break;
-    FT(0) FT(1) FT(2) FT(3) FT(4) FT(5) FT(6) FT(7) FT(8) FT(9) FT(10) FT(11)
+    FT(0) FT(1) FT(2) FT(3) FT(4) FT(5) FT(6) FT(7) FT(8) FT(9) FT(10) FT(11) FT(12)
 #undef FT
     default:
       break;
@@ -1349,6 +1349,8 @@ __global__ __launch_bounds__(64) void k_type2_tail(DevTables T, DevJob J, T2Args
 // order, by the assembly.
 constexpr int kFb2PerLane = 1;
 constexpr int kFb2Cnt = 52;  // fbc[kFb2Cnt + c]: triplets queued for the type-2 fallback, class c
+constexpr int kFbcSlots = 80;  // queue counters per job
+static_assert(ECP_MAX_LAB < 16 && kFb2Cnt + kMaxLMB < kFbcSlots, "queue counter slots");
 
 template <int LMB>
 __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, const DevJob &J,
@@ -1619,8 +1621,8 @@ struct T1Set {
   double *qfb;
 };
 
-#define ECP_CASES_LAB(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
-#define ECP_CASES_LMB(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
+#define ECP_CASES_LAB(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#define ECP_CASES_LMB(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
 template <int MAXLAB, int MODE>
 __global__ __launch_bounds__(64) void k_type1_fused(DevTables T, DevJob J, T1Set S, ClassMap cm) {
@@ -2301,6 +2303,7 @@ struct ecp_dev {
   char *pin_out = nullptr;
   size_t pin_out_cap = 0;
   double *dfac_dev = nullptr, *fac_dev = nullptr;
+  int lds_max = 64 * 1024;  // LDS per workgroup of the device (160 KiB on gfx950)
   hipEvent_t ev[8] = {};
   float last_ms[8] = {};
   int nev = 0;
@@ -2308,7 +2311,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 15, 100000, 8192, 2, 2, 0, 2048, 0, 1};
+  ecp_tuning tune{8, 8, 100000, 8192, 2, 2, 0, 2048, 0, 1};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph[kSpanGraphs] = {};
   hipGraphExec_t gexec[kSpanGraphs] = {};
@@ -2416,6 +2419,13 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
     snprintf(err, errlen, "no HIP device available (%s)", hipGetErrorString(e));
     delete d;
     return nullptr;
+  }
+  {
+    int dev = 0, v = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return fail("hipGetDevice", e);
+    if ((e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) != hipSuccess)
+      return fail("hipDeviceGetAttribute", e);
+    d->lds_max = v;
   }
   if ((e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking)) != hipSuccess) return fail("hipStreamCreate", e);
   for (int i = 0; i < kWorkers; i++) {
@@ -2692,6 +2702,17 @@ void launch_fb1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, 
   }
 }
 
+// the type-2 fallback classes of cm in one grid; like the type-1 kernels,
+// three instantiations bound the Bessel orders the grid holds: up to 7, 11
+// and 12
+void launch_fb2_fused(ecp_dev *d, hipStream_t s, const T2Set &S2, const ClassMap &cm) {
+  const Prepared &P = d->P;
+  const int nb = cm.bbeg[cm.n];
+  if (P.max_cls2 > 11) k_fb2_fused<12><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
+  else if (P.max_cls2 > 7) k_fb2_fused<11><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
+  else k_fb2_fused<7><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
+}
+
 // waves per (triplet, pair) of the type-2 fallback (fb2_split; 0: automatic,
 // 3 for jobs of up to kFb2SplitMaxItems (triplet, pair) items, where the
 // fallback's latency counts, 1 for larger, throughput-bound ones)
@@ -2779,7 +2800,7 @@ void enqueue_t1_class(ecp_dev *d, hipStream_t ws, int c) {
   case c:     \
     enqueue_t1_class_t<c>(d, ws); \
     break;
-    T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10)
+    T1(0) T1(1) T1(2) T1(3) T1(4) T1(5) T1(6) T1(7) T1(8) T1(9) T1(10) T1(11) T1(12)
 #undef T1
   }
 }
@@ -2830,8 +2851,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       });
       if (cm2.n > 0) {
         Span sp(d, s, FAM_FB2);
-        if (P.max_cls2 > 7) k_fb2_fused<11><<<cm2.bbeg[cm2.n], 64, 0, s>>>(T, J, S2, cm2);
-        else k_fb2_fused<7><<<cm2.bbeg[cm2.n], 64, 0, s>>>(T, J, S2, cm2);
+        launch_fb2_fused(d, s, S2, cm2);
       }
     }
     {
@@ -2910,7 +2930,9 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     tailcnt[c] = coop ? P.cls_count[c] : 0;
     tail = tail || (coop && P.cls_count[c] > 0);
   }
-  const bool big1 = P.max_lab > 6;
+  // the fused kernels hold the registers of their largest class: three
+  // instantiations, classes up to 6, 10 and 12
+  const int tier1 = P.max_lab > 10 ? 2 : P.max_lab > 6 ? 1 : 0;
   // speculative fallback: with a tail, the large-grid fallback starts after
   // the head on every entry the head left open, beside the tail, instead of
   // after the tail on the failed ones (most of the entries open after the
@@ -2927,7 +2949,8 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     (void)hipEventRecordWithFlags(d->span[d->span_k][0], w[0], d->capturing ? hipEventRecordExternal : 0);
     {
       Span sp(d, w[0], FAM_TYPE1);
-      if (big1) k_type1_fused<10, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
+      if (tier1 == 2) k_type1_fused<12, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
+      else if (tier1 == 1) k_type1_fused<10, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
       else k_type1_fused<6, 1><<<cmh.bbeg[cmh.n], 64, 0, w[0]>>>(T, J, S, cmh);
     }
     if (spec) (void)hipEventRecord(d->h1done, w[0]);
@@ -2935,7 +2958,8 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       const ClassMap cmt = class_map(ECP_MAX_LAB + 1, tailcnt,
                                      [&](int c) { return std::min(tailcnt[c], tu.t1_tail_grid); });
       Span sp(d, w[0], FAM_T1TAIL);
-      if (big1) k_t1tail_fused<10><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
+      if (tier1 == 2) k_t1tail_fused<12><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
+      else if (tier1 == 1) k_t1tail_fused<10><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
       else k_t1tail_fused<6><<<cmt.bbeg[cmt.n], 64, 0, w[0]>>>(T, J, S, cmt);
     }
     (void)hipEventRecordWithFlags(d->span[d->span_k][1], w[0], d->capturing ? hipEventRecordExternal : 0);
@@ -2948,8 +2972,10 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     const ClassMap cmf = class_map(ECP_MAX_LAB + 1, P.cls_count, [&](int c) { return std::min(P.cls_count[c], 4096); });
     {
       Span sp(d, w[1], FAM_FB1);
-      if (big1 && spec) k_fb1_fused<10, true><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
-      else if (big1) k_fb1_fused<10, false><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      if (tier1 == 2 && spec) k_fb1_fused<12, true><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      else if (tier1 == 2) k_fb1_fused<12, false><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      else if (tier1 == 1 && spec) k_fb1_fused<10, true><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
+      else if (tier1 == 1) k_fb1_fused<10, false><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
       else if (spec) k_fb1_fused<6, true><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
       else k_fb1_fused<6, false><<<cmf.bbeg[cmf.n], 64, 0, w[1]>>>(T, J, S, cmf);
     }
@@ -2983,8 +3009,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     });
     if (cm2.n > 0) {
       Span sp(d, w[3], FAM_FB2);
-      if (P.max_cls2 > 7) k_fb2_fused<11><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
-      else k_fb2_fused<7><<<cm2.bbeg[cm2.n], 64, 0, w[3]>>>(T, J, S2, cm2);
+      launch_fb2_fused(d, w[3], S2, cm2);
     }
     (void)hipStreamWaitEvent(w[2], d->angdone, 0);
     {
@@ -3143,7 +3168,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     return o;
   };
   Ly.d_zero = dv_off;
-  Ly.d_fbc = dv_take(4 * 64);
+  Ly.d_fbc = dv_take(4 * kFbcSlots);
   Ly.d_gflag = dv_take(4 * (size_t)P.ngroups);
   Ly.d_stat = dv_take(2 * (size_t)nt);
   Ly.d_t1fb = dv_take((size_t)nt);
@@ -3223,9 +3248,9 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
       P.asm1_lds4 = (size_t)kAsmPack * a4.lds_group * sizeof(double);
     }
     P.asm2_lds = (size_t)end2 * sizeof(double);
-    if (std::max(P.asm1_lds, P.asm2_lds) > 64 * 1024) {
-      snprintf(err, errlen, "assembly needs %zu bytes of LDS (l too high for this build)",
-               std::max(P.asm1_lds, P.asm2_lds));
+    if (std::max(P.asm1_lds, P.asm2_lds) > (size_t)d->lds_max) {
+      snprintf(err, errlen, "assembly needs %zu bytes of LDS, the device has %d per workgroup",
+               std::max(P.asm1_lds, P.asm2_lds), d->lds_max);
       return -1;
     }
   }
@@ -3335,8 +3360,8 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.qopen = (unsigned long long *)(A + Ly.d_qopen);
   P.qfbf = (unsigned long long *)(A + Ly.d_qfbf);
   P.fbq1 = (int *)(A + Ly.d_fbq1);
-  P.fbc = (int *)(A + Ly.d_fbc);  // [0..10] type-1 fallback queues, [16..27] type-2, [32..42] type-1 tails,
-                                  // [48] type-2 tail
+  P.fbc = (int *)(A + Ly.d_fbc);  // [0..12] type-1 fallback queues, [32..44] type-1 tails, [48] type-2
+                                  // tail, [kFb2Cnt + 0..12] type-2 fallback queues
   P.stat = (unsigned char *)(A + Ly.d_stat);
   P.t1fb = (unsigned char *)(A + Ly.d_t1fb);
   P.t2fb = (unsigned char *)(A + Ly.d_t2fb);

@@ -280,7 +280,7 @@ int ecp_job_build(const ecp_mol *mols, int nmol, int deriv, const ecp_static *st
     return -1;
   }
   if (maxLBS + deriv > ECP_MAX_LBS) {
-    set_err(err, errlen, "basis angular momentum plus derivative order exceeds this build (at most 5)");
+    set_err(err, errlen, "basis angular momentum plus derivative order exceeds this build (at most 6)");
     free(dims);
     return -1;
   }

@@ -1,15 +1,18 @@
 """The largest angular momenta this build evaluates: basis l plus derivative
-order up to 5, so type-1 classes up to lambda = la + lb = 10 (the kernels'
-largest template), with L = 6 (semi-local channels l = 0..5, the local
-channel l = 6), inside the reference's valid domain (basis l + n <= L - 1).
+order up to 6, so type-1 classes up to lambda = la + lb = 12 (the kernels'
+largest template) and Bessel orders up to L - 1 + 6 = 12, with L = 7
+(semi-local channels l = 0..6, the local channel l = 7), inside the
+reference's valid domain (basis l + n <= L - 1).  The reference sizes its
+tables from the inputs (libecp.c:129-149, 176-185) and has no fixed cap.
 
-Two molecules: h shells (l = 5) at n = 0 (classes up to lambda = 10), and
-g shells (l = 4) at n = 1 (one shell shifted to l = 5: lambda up to 9).
-On the CPU the oracle is bit-identical to the reference on both (where
-oracle/_ref is built); on the device every block is within 1e-10 absolute
-+ 1e-8 relative of the oracle, widened by the reference's own rounding
-floor at these l (2 |ref - ref_fma|, ref_fma the oracle compiled with FMA
-contraction).
+Four molecules: i shells (l = 6) at n = 0 (classes up to lambda = 12), h
+shells (l = 5) at n = 1 (one shell shifted to l = 6: lambda up to 11), and
+under L = 6 h shells at n = 0 (lambda up to 10) and g shells at n = 1
+(lambda up to 9).  On the CPU the oracle is bit-identical to the reference
+on all four (where oracle/_ref is built); on the device every block is
+within 1e-10 absolute + 1e-8 relative of the oracle, widened by the
+reference's own rounding floor at these l (2 |ref - ref_fma|, ref_fma the
+oracle compiled with FMA contraction).
 """
 import ctypes as C
 import os
@@ -25,6 +28,7 @@ GEOM = [0.0, 0.0, 0.0, 1.6, 0.4, -0.7, -0.9, 1.3, 0.6]
 ECP6 = [(0, [(2, 9.0, 3.1), (2, 1.6, 0.8)]), (1, [(2, -3.0, 2.7)]), (2, [(2, 1.2, 1.9)]),
         (3, [(2, -0.8, 1.4)]), (4, [(2, 0.5, 1.1)]), (5, [(2, -0.3, 0.9)]), (6, [(2, 0.2, 1.3)])]
 
+ECP7 = ECP6 + [(7, [(2, 0.15, 1.2)])]
 
 def _shell(l, *prims):
     return (l, list(prims))
@@ -44,7 +48,22 @@ def g_shells():
     return cases._mol(GEOM, basis, [ECP6, None, ECP6])
 
 
-MOLS = {"h_shells_n0": (h_shells, 0, 10), "g_shells_n1": (g_shells, 1, 9)}
+def i_shells():
+    basis = [[_shell(0, (4.0, 0.5), (0.6, 0.6)), _shell(3, (1.8, 0.7)), _shell(6, (1.1, 0.6), (0.5, 0.5))],
+             [_shell(1, (2.2, 0.6)), _shell(4, (1.0, 0.8)), _shell(6, (0.8, 0.7))],
+             [_shell(2, (1.5, 0.9)), _shell(5, (0.7, 0.6))]]
+    return cases._mol(GEOM, basis, [ECP7, None, ECP7])
+
+
+def h_shells_l7():
+    basis = [[_shell(0, (4.0, 0.5), (0.6, 0.6)), _shell(5, (1.2, 0.6), (0.5, 0.5))],
+             [_shell(2, (1.6, 0.7)), _shell(5, (0.9, 0.7))],
+             [_shell(1, (1.5, 0.9)), _shell(3, (0.8, 0.6))]]
+    return cases._mol(GEOM, basis, [ECP7, None, ECP7])
+
+
+MOLS = {"h_shells_n0": (h_shells, 0, 10), "g_shells_n1": (g_shells, 1, 9),
+        "i_shells_n0": (i_shells, 0, 12), "h_shells_n1": (h_shells_l7, 1, 11)}
 
 
 @pytest.mark.skipif(not E.have_ref(), reason="oracle/_ref not built (reference sources absent)")
