@@ -55,12 +55,12 @@ def test_init_rejects_derivative_order_above_two():
 
 
 def test_derivative_order_limited_by_basis_l():
-    # basis up to d (l = 2): n = 2 reaches l = 4 and is accepted; an h shell
-    # (l = 5) leaves no room for a shift
+    # basis up to d (l = 2): n = 2 reaches l = 4 and is accepted; an i shell
+    # (l = 6, the largest basis l) leaves no room for a shift
     m = plain_small(0)
     keys, _ = screened(m, n=2)
     assert len(keys) > 0
-    m.lBS[-1] = 5
+    m.lBS[-1] = 6
     lib = native.lib()
     h = handle(m, n=1)
     assert h
