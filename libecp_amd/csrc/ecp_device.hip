@@ -2056,9 +2056,11 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
 // entries (small-grid results, or the sum of the per-pair fallback results)
 // are placed in LDS and contracted with the angular tables of both atoms
 // (type2.c:583-623), then the block is shifted (util.c:246-334).
-// NB, NM > 0: the angular factors of atom B for every lambda2 (at most NB
+// NB > 0: the angular factors of atom B for every lambda2 (at most NB
 // values) and m (at most NM) are held in registers for each l, those of atom
-// A per lambda1; NB = 0 reads both from the table in the inner loop.
+// A per lambda1; NB = 0: both read for each (lambda1, lambda2), with NM > 1
+// the 2 l + 1 values of each loaded together (NM >= 2 l + 1 of every
+// projector l).
 template <int NB, int NM, int TPB = kAsmThreads>
 __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
                                                       const unsigned char *__restrict__ status,
@@ -2140,6 +2142,25 @@ __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
             }
           }
         }
+      } else if constexpr (NM > 1) {
+        // all 2 l + 1 factor pairs of an angular factor loaded together (NM
+        // >= 2 l + 1), then summed in m order
+        for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
+          for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
+            const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
+            const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
+            double x[NM], y[NM];
+#pragma unroll
+            for (int m = 0; m < NM; m++) {
+              x[m] = m < nm ? pa[m * J.ang_c] : 0.0;
+              y[m] = m < nm ? pb[m * J.ang_c] : 0.0;
+            }
+            double factor = 0.0;
+#pragma unroll
+            for (int m = 0; m < NM; m++)
+              if (m < nm) factor += x[m] * y[m];
+            tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
+          }
       } else {
         for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
           for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
@@ -2610,7 +2631,9 @@ void launch_type2(ecp_dev *d, hipStream_t s, bool fused) {
 template <int TPB>
 void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache) {
   const Prepared &P = d->P;
-  if (!cache)
+  if (!cache && P.max_L <= 4)
+    k_asm2<0, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+  else if (!cache)
     k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
   else if (P.max_lbs <= 3 && P.max_L <= 4)
     k_asm2<4, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
