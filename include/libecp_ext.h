@@ -171,8 +171,11 @@ int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples);
    large-grid fallback; 0, the default: 3 for jobs of up to a million
    (triplet, pair) items, 1 for larger ones).  "t1_spec" (fused launches: 1, the default, starts
    the type-1 large-grid fallback beside the small-grid tail on every entry
-   the head left open; 0 after the tail on the failed entries).  Results do
-   not depend on them.  0, or -1 for an unknown name. */
+   the head left open; 0 after the tail on the failed entries).  "t2_spec"
+   (fused launches with a type-2 head: 1 starts the type-2 large-grid
+   fallback beside the tail on every entry the head left open; 0, the
+   default, after the tail on the failed entries).  Results do not depend on
+   them.  0, or -1 for an unknown name. */
 int libecp_batch_set_param(libecp_batch *b, const char *name, int value);
 int libecp_batch_get_param(libecp_batch *b, const char *name, int *value);
 /* Autotuner: times the batch (graph replays, `repeats` per candidate) over

@@ -797,6 +797,7 @@ static int *tuning_field(ecp_tuning *t, const char *name) {
   if (!strcmp(name, "t2_tail_grid")) return &t->t2_tail_grid;
   if (!strcmp(name, "fb2_split")) return &t->fb2_split;
   if (!strcmp(name, "t1_spec")) return &t->t1_spec;
+  if (!strcmp(name, "t2_spec")) return &t->t2_spec;
   return NULL;
 }
 

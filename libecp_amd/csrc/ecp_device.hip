@@ -1215,25 +1215,30 @@ struct T2Args {
   int *tq, *tcount;  // entries left open by the head
   int *tclaim;                  // per triplet: queued for the fallback
   const unsigned char *tcls2;   // per triplet: its fallback class max(la, lb) + L - 1
+  int spec;                     // the fallback takes every entry the head leaves open
 };
 
+// An entry for the large-grid fallback: gflag[g] counts the group's
+// entries, listed (in any order) at flist[eoff ...]; the triplet's first one
+// queues the triplet, whose listed entries of all groups the fallback takes
+// together
+__device__ __forceinline__ void t2_enlist(const T2Args &A, int gid, int g, const T2Group &G) {
+  const int k = atomicAdd(&A.gflag[g], 1);
+  A.flist[G.eoff + k] = gid - G.eoff;
+  if (k == 0 && atomicExch(&A.tclaim[G.trip], 1) == 0) {
+    const int c = A.tcls2[G.trip];
+    const int slot = atomicAdd(&A.fbcount[c], 1);
+    A.fbq[(size_t)c * A.qstride + slot] = G.trip;
+    A.fbflag[G.trip] = 1;
+  }
+}
+
+// A decided entry: its small-grid result and failure flag; a failed entry
+// goes to the fallback, unless the fallback already took it speculatively
 __device__ __forceinline__ void t2_store(const T2Args &A, int gid, int g, const T2Group &G, int st, double res) {
   A.Tout[gid] = res;
   A.tfail[gid] = st == 0 ? 0 : 1;
-  if (st != 0) {
-    // gflag[g] counts the group's failed entries, listed (in any order) at
-    // flist[eoff ...]; the first one queues the group for the fallback
-    const int k = atomicAdd(&A.gflag[g], 1);
-    A.flist[G.eoff + k] = gid - G.eoff;
-    // the triplet's first failed entry queues the triplet: the fallback
-    // takes all its failed groups together
-    if (k == 0 && atomicExch(&A.tclaim[G.trip], 1) == 0) {
-      const int c = A.tcls2[G.trip];
-      const int slot = atomicAdd(&A.fbcount[c], 1);
-      A.fbq[(size_t)c * A.qstride + slot] = G.trip;
-      A.fbflag[G.trip] = 1;
-    }
-  }
+  if (st != 0 && !A.spec) t2_enlist(A, gid, g, G);
 }
 
 // One lane per T entry.  With head < the level count, entries still open
@@ -1250,6 +1255,7 @@ __global__ __launch_bounds__(128) void k_type2(DevTables T, DevJob J, T2Args A, 
   const int st = t2_entry(T, J, G, code, min(head, T.s93.nlevels), &res);
   if (st == 2) {
     A.tq[atomicAdd(A.tcount, 1)] = gid;
+    if (A.spec) t2_enlist(A, gid, g, G);
     return;
   }
   t2_store(A, gid, g, G, st, res);
@@ -1360,7 +1366,7 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
                                          unsigned char *__restrict__ status, const int *__restrict__ gcount,
                                          const int *__restrict__ flist, const int *__restrict__ tgfirst,
                                          int nsplit, double *s_r, double *s_w, double *s_cu, double *s_ex,
-                                         double *s_ka, double *s_kb) {
+                                         double *s_ka, double *s_kb, unsigned char *__restrict__ fbf2) {
   const int lane = threadIdx.x;
   // items (split, triplet, pair), split-major so that the grid-stride loop
   // spreads every split over all workgroups; split k takes the passes k,
@@ -1578,6 +1584,7 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
       for (int j = 0; j < kFb2PerLane; j++) {
         if (ent[j] < 0) continue;
         if (done & (1u << j)) fbT[(size_t)eo[j] * J.maxPairs + pair] = res[j];
+        else if (fbf2) fbf2[eo[j]] = 1;
         else status[2 * t + 1] = 1;
       }
     }
@@ -1690,6 +1697,7 @@ struct T2Set {
   int nt;
   const int *gcount, *flist, *tgfirst;
   int nsplit;
+  unsigned char *fbf2;  // speculative fallback: per entry, the large grid failed (nullptr: off)
 };
 
 template <int MAXLMB>
@@ -1703,7 +1711,7 @@ __global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S
   case L:                                                                                                      \
     if constexpr (L <= MAXLMB)                                                                                 \
       fb2_body<L>(bid, nb, T, J, S.groups, S.tmpl, S.tf, S.fbq2 + (size_t)L * S.nt, S.fbc + kFb2Cnt + L, S.fbT, \
-                  S.stat, S.gcount, S.flist, S.tgfirst, S.nsplit, s_r, s_w, s_cu, s_ex, s_ka, s_kb);            \
+                  S.stat, S.gcount, S.flist, S.tgfirst, S.nsplit, s_r, s_w, s_cu, s_ex, s_ka, s_kb, S.fbf2);    \
     break;
     ECP_CASES_LMB(X)
 #undef X
@@ -1726,6 +1734,7 @@ struct AsmArgs {
   const int *tmpl;
   const double *Tout;
   const unsigned char *tfail;
+  const unsigned char *fbf2;  // speculative type-2 fallback: per entry, the large grid failed (nullptr: off)
   const double *fbT;
   const double *fac, *dfac;
   double *blocks;
@@ -1945,19 +1954,28 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
   const ecp_centre Cn = J.cent[tr.centre];
   const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
   const int la = tr.la, lb = tr.lb, lab = la + lb;  // shifted for derivatives
-  if constexpr (G == 1) {
-    if (A.qfb) {
-      // speculative fallback: an integral failed where the small grid and
-      // the large grid both failed (type1.c:182-190, code 1)
-      bool bad = false;
+  if (A.qfb) {
+    // speculative fallback: an integral failed where the small grid and
+    // the large grid both failed (type1.c:182-190, code 1)
+    bool bad = false;
+    if (active)
       for (int k = lid; k < Sa.K * Sb.K; k += W) {
         const int ug = A.t1first[t] + k;
         bad = bad || (A.qfail[ug] & A.qfbf[ug]) != 0ull;
       }
+    if constexpr (G == 1) {
       if (wave_any(bad)) {
         if (lid == 0) status[2 * t] = 1;
         return;
       }
+    } else {
+      // the verdict of the lane group's W lanes
+      const unsigned long long m = __ballot(bad);
+      if ((m >> (grp * W)) & ((1ull << W) - 1ull)) {
+        if (lid == 0) status[2 * t] = 1;
+        active = false;
+      }
+      if (!wave_any(active)) return;
     }
   }
   const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
@@ -2063,7 +2081,7 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
 // projector l).
 template <int NB, int NM, int TPB = kAsmThreads>
 __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
-                                                      const unsigned char *__restrict__ status,
+                                                      unsigned char *__restrict__ status,
                                                       const unsigned char *__restrict__ fbflag, int filter) {
   extern __shared__ double lds[];
   double *blk = lds + A.lds_blk;
@@ -2082,13 +2100,26 @@ __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
   const double rBC[3] = {rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
   const V3 vA{rAC[0], rAC[1], rAC[2]}, vB{rBC[0], rBC[1], rBC[2]};
   const int D = T.ijkDim;
+  const int g0 = A.tgfirst[t];
+  if (A.fbf2 && filter == 2) {
+    // speculative fallback: the triplet failed where an entry failed on the
+    // small grid and on the large grid (type2.c:417-528, code 1)
+    bool bad = false;
+    for (int l = 0; l < Cn.L; l++) {
+      const T2Group G = A.groups[g0 + l];
+      for (int e = threadIdx.x; e < G.ne; e += blockDim.x) bad = bad || (A.tfail[G.eoff + e] && A.fbf2[G.eoff + e]);
+    }
+    if (__syncthreads_or(bad)) {
+      if (threadIdx.x == 0) status[2 * t + 1] = 1;
+      return;
+    }
+  }
 
   for (int i = threadIdx.x; i < nca * ncb; i += blockDim.x) blk[i] = 0.0;
   __syncthreads();
   const int nlm2 = J.angL * J.angL;
   const double *angA = J.ang + Cn.ang_off + (size_t)(Sa.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
   const double *angB = J.ang + Cn.ang_off + (size_t)(Sb.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
-  const int g0 = A.tgfirst[t];
   for (int l = 0; l < Cn.L; l++) {
     const T2Group G = A.groups[g0 + l];
     const int d1 = la + l + 1, d2 = lb + l + 1, d3 = lab + 1;
@@ -2235,7 +2266,7 @@ struct JobLayout {
   size_t i_atoms = 0, i_amaxl = 0, i_sh = 0, i_pa = 0, i_pd = 0, i_cent = 0, i_gauss = 0, i_win = 0, i_trip = 0,
          i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, i_tcls2 = 0,
          in_bytes = 0;
-  size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_tclaim = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
+  size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_tclaim = 0, d_fbf2 = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
          d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_qfb = 0, d_qopen = 0, d_qfbf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0, deriv = 0;
 };
@@ -2267,6 +2298,7 @@ struct Prepared {
   int tcount2[kMaxLMB + 1] = {0};  // triplets per type-2 fallback class
   int max_cls2 = 0;
   int *tclaim = nullptr;
+  unsigned char *fbf2 = nullptr;  // per T entry: the speculative fallback failed on the large grid
   const unsigned char *tcls2 = nullptr;
   const int *tgf = nullptr;
   int max_lbs = 0, max_L = 0;  // largest basis l and ECP local label of the job
@@ -2304,6 +2336,7 @@ struct ecp_dev {
   hipStream_t workers[kWorkers] = {};
   hipEvent_t fork = nullptr, t2done = nullptr, t1done = nullptr, angdone = nullptr, join[kWorkers] = {};
   hipEvent_t h1done = nullptr;  // type-1 head finished (speculative fallback)
+  hipEvent_t h2done = nullptr;  // type-2 head finished (speculative fallback)
   // the type-1 small-grid quadrature (head and tail launches), recorded on
   // its stream: event pair k belongs to captured graph k (replays cycle over
   // the graphs, so the last kSpanGraphs replays can be read); the eager
@@ -2332,7 +2365,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 8, 100000, 8192, 2, 2, 0, 2048, 0, 1};
+  ecp_tuning tune{8, 8, 100000, 8192, 2, 2, 0, 2048, 0, 1, 0};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph[kSpanGraphs] = {};
   hipGraphExec_t gexec[kSpanGraphs] = {};
@@ -2460,6 +2493,7 @@ extern "C" ecp_dev *ecp_dev_create(const ecp_static *st, char *err, size_t errle
   if ((e = hipEventCreateWithFlags(&d->t1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->angdone, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   if ((e = hipEventCreateWithFlags(&d->h1done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
+  if ((e = hipEventCreateWithFlags(&d->h2done, hipEventDisableTiming)) != hipSuccess) return fail("hipEventCreate", e);
   for (int k = 0; k < kSpanGraphs; k++)
     for (int i = 0; i < 2; i++)
       if ((e = hipEventCreate(&d->span[k][i])) != hipSuccess) return fail("hipEventCreate", e);
@@ -2576,6 +2610,7 @@ extern "C" void ecp_dev_destroy(ecp_dev *d) {
   if (d->t1done) (void)hipEventDestroy(d->t1done);
   if (d->angdone) (void)hipEventDestroy(d->angdone);
   if (d->h1done) (void)hipEventDestroy(d->h1done);
+  if (d->h2done) (void)hipEventDestroy(d->h2done);
   for (int k = 0; k < kSpanGraphs; k++)
     for (int i = 0; i < 2; i++)
       if (d->span[k][i]) (void)hipEventDestroy(d->span[k][i]);
@@ -2605,18 +2640,26 @@ namespace {
 // waves on the device) or the compacted lane-serial tail when it is large.
 // type-2 small grid: the lane-per-entry kernel, split into head and
 // compacted tail when t2_head is below the level count
-void launch_type2(ecp_dev *d, hipStream_t s, bool fused) {
+// PS93 levels of the type-2 head (t2_head; 0: automatic -- 6 when the small
+// grid is on the critical chain (fused launches), none in large per-class
+// jobs)
+int t2_head_levels(const ecp_dev *d, bool fused) {
+  return d->tune.t2_head > 0 ? d->tune.t2_head : (fused ? 6 : 15);
+}
+
+// spec: the head enlists every entry it leaves open for the fallback, which
+// can start at `mid` (recorded after the head) beside the tail
+void launch_type2(ecp_dev *d, hipStream_t s, bool fused, bool spec = false, hipEvent_t mid = nullptr) {
   const Prepared &P = d->P;
   if (P.ntot2 <= 0) return;
   T2Args A{P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + kFb2Cnt, P.nt,
-           P.t2fb, P.flist, P.tq2, P.fbc + 48, P.tclaim, P.tcls2};
-  // 0: automatic -- a head of 6 levels when the small grid is on the
-  // critical chain (fused launches), none in large per-class jobs
-  const int head = d->tune.t2_head > 0 ? d->tune.t2_head : (fused ? 6 : 15);
+           P.t2fb, P.flist, P.tq2, P.fbc + 48, P.tclaim, P.tcls2, spec ? 1 : 0};
+  const int head = t2_head_levels(d, fused);
   {
     Span sp(d, s, FAM_TYPE2);
     k_type2<<<(P.ntot2 + 127) / 128, 128, 0, s>>>(d->T, P.J, A, head);
   }
+  if (mid) (void)hipEventRecord(mid, s);
   if (head < d->T.s93.nlevels) {
     Span sp(d, s, FAM_TYPE2);
     k_type2_tail<<<std::min((P.ntot2 + 15) / 16, d->tune.t2_tail_grid), 64, 0, s>>>(d->T, P.J, A);
@@ -2629,18 +2672,20 @@ void launch_type2(ecp_dev *d, hipStream_t s, bool fused) {
 // cache costs occupancy: it pays where the assembly is on the critical path
 // of a small job (fused launches), not in large throughput-bound ones.
 template <int TPB>
-void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache) {
+void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache, bool spec) {
   const Prepared &P = d->P;
+  AsmArgs aa = P.aa;
+  aa.fbf2 = spec ? P.fbf2 : nullptr;
   if (!cache && P.max_L <= 4)
-    k_asm2<0, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+    k_asm2<0, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
   else if (!cache)
-    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
   else if (P.max_lbs <= 3 && P.max_L <= 4)
-    k_asm2<4, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+    k_asm2<4, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
   else if (P.max_lbs <= 4 && P.max_L <= 5)
-    k_asm2<5, 9, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+    k_asm2<5, 9, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
   else
-    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, P.aa, P.stat, P.t2fb, filter);
+    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
 }
 
 // type-1 assembly of the triplets tcls[b .. e) (class-ordered), one per
@@ -2660,9 +2705,16 @@ void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack,
     aa.qfb = P.qfb;
   }
   const int m = pack ? std::min(std::max(b, P.tcls_begin[kAsmPackLab + 1]), e) : b;
-  if (m > b)
+  if (m > b) {
+    AsmArgs a4 = P.aa4;
+    if (spec) {
+      a4.qfail = P.qf;
+      a4.qfbf = P.qfbf;
+      a4.qfb = P.qfb;
+    }
     k_asm1<kAsmPack, kAsmPackLab><<<(m - b + kAsmPack - 1) / kAsmPack, kAsmThreads, P.asm1_lds4, s>>>(
-        d->T, P.J, P.aa4, P.stat, P.tcls + b, m - b, P.t1fb, filter);
+        d->T, P.J, a4, P.stat, P.tcls + b, m - b, P.t1fb, filter);
+  }
   if (e <= m) return;
   const dim3 g(e - m);
   switch (cls) {
@@ -2680,13 +2732,15 @@ void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack,
 // The assembly of the triplets with large-grid work (filter 2) comes last
 // and covers few triplets: with fused launches it runs 4 waves per triplet,
 // so that each triplet's elements finish sooner.
-void launch_asm2(ecp_dev *d, hipStream_t s, int filter, bool fused) {
+// spec: the fallback ran speculatively; a triplet failed where an entry
+// failed on both grids
+void launch_asm2(ecp_dev *d, hipStream_t s, int filter, bool fused, bool spec = false) {
   const int mode = d->tune.asm2_cache;
   const bool cache = mode == 1 || (mode == 2 && fused);
   if (fused && filter == 2)
-    launch_asm2_tpb<4 * kAsmThreads>(d, s, filter, cache);
+    launch_asm2_tpb<4 * kAsmThreads>(d, s, filter, cache, spec);
   else
-    launch_asm2_tpb<kAsmThreads>(d, s, filter, cache);
+    launch_asm2_tpb<kAsmThreads>(d, s, filter, cache, spec);
 }
 
 template <int LAB>
@@ -3022,11 +3076,16 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (P.ntot2 > 0) {
     // likewise the fallback first (it needs no angular tables), then the
     // assembly beside it, then the assembly that waits for the fallback
-    launch_type2(d, w[2], true);
+    // t2_spec: the fallback starts after the head, beside the tail, on every
+    // entry the head left open (most of them fail); the assembly then takes
+    // the fallback's result where the tail failed too
+    const bool spec2 = tu.t2_spec != 0 && t2_head_levels(d, true) < T.s93.nlevels;
+    launch_type2(d, w[2], true, spec2, spec2 ? d->h2done : nullptr);
     (void)hipEventRecord(d->t2done, w[2]);
-    (void)hipStreamWaitEvent(w[3], d->t2done, 0);
+    (void)hipStreamWaitEvent(w[3], spec2 ? d->h2done : d->t2done, 0);
     const int nsplit = fb2_split(d, true);
     T2Set S2{P.grp, d->tmpl_dev, P.tf, P.fbq2, P.fbc, P.fbT, P.stat, P.nt, P.gflag, P.flist, P.tgf, nsplit};
+    S2.fbf2 = spec2 ? P.fbf2 : nullptr;
     const ClassMap cm2 = class_map(kMaxLMB + 1, P.tcount2, [&](int c) {
       return (int)std::min<long>((long)P.tcount2[c] * P.maxPairs * nsplit, 8192);
     });
@@ -3039,10 +3098,11 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       Span sp(d, w[2], FAM_ASM);
       launch_asm2(d, w[2], 1, true);
     }
+    if (spec2) (void)hipStreamWaitEvent(w[3], d->t2done, 0);
     (void)hipStreamWaitEvent(w[3], d->angdone, 0);
     {
       Span sp(d, w[3], FAM_ASM);
-      launch_asm2(d, w[3], 2, true);
+      launch_asm2(d, w[3], 2, true, spec2);
     }
   }
   for (int i = 0; i < kWorkers; i++) {
@@ -3197,6 +3257,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_t1fb = dv_take((size_t)nt);
   Ly.d_t2fb = dv_take((size_t)nt);
   Ly.d_tclaim = dv_take(4 * (size_t)nt);
+  Ly.d_fbf2 = dv_take((size_t)ntot2);
   Ly.d_zero_end = dv_off;
   Ly.d_UL = dv_take(8 * job->nUL);
   Ly.d_Ut = dv_take(8 * job->nUt);
@@ -3397,6 +3458,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.tq2 = (int *)(A + Ly.d_tq2);
   P.fbq2 = (int *)(A + Ly.d_fbq2);
   P.tclaim = (int *)(A + Ly.d_tclaim);
+  P.fbf2 = (unsigned char *)(A + Ly.d_fbf2);
   P.tcls2 = (const unsigned char *)(A + Ly.i_tcls2);
   P.tgf = (const int *)(A + Ly.i_tgf);
   P.fbT = (double *)(A + Ly.d_fbT);
@@ -3516,6 +3578,7 @@ namespace {
 __global__ __launch_bounds__(256) void k_fb_flops(DevJob J, const T1Unit *__restrict__ units,
                                                   const unsigned long long *__restrict__ qf, int nunits,
                                                   const T2Group *__restrict__ groups, const int *__restrict__ gflag,
+                                                  const int *__restrict__ flist, const unsigned char *__restrict__ tfail,
                                                   int ngroups, int N, double *out) {
   double s1 = 0.0, s2 = 0.0;
   const int stride = gridDim.x * blockDim.x;
@@ -3526,9 +3589,12 @@ __global__ __launch_bounds__(256) void k_fb_flops(DevJob J, const T1Unit *__rest
     s1 += ecp_fb1_flops(tr.la + tr.lb, f, N);
   }
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
-    const int f = gflag[g];
-    if (!f) continue;
+    // the group's failed entries among those listed for the fallback (a
+    // speculative fallback also lists entries the small grid then decided)
     const T2Group G = groups[g];
+    int f = 0;
+    for (int k = 0; k < gflag[g]; k++) f += tfail[G.eoff + flist[G.eoff + k]];
+    if (!f) continue;
     const ecp_triplet tr = J.trip[G.trip];
     const ecp_centre C = J.cent[tr.centre];
     int ng = 0;
@@ -3579,7 +3645,8 @@ extern "C" int ecp_dev_profile(ecp_dev *d, int *launches, double *ms, double *fl
   ECP_CHECK(hipMemsetAsync(d->fbf, 0, 2 * sizeof(double), d->stream));
   const int nunits = P.cls_begin[ECP_MAX_LAB + 1];
   const int nb = std::max(1, std::min(1024, (std::max(nunits, P.ngroups) + 255) / 256));
-  k_fb_flops<<<nb, 256, 0, d->stream>>>(P.J, P.u1, P.qf, nunits, P.grp, P.gflag, P.ngroups, d->T.lorder, d->fbf);
+  k_fb_flops<<<nb, 256, 0, d->stream>>>(P.J, P.u1, P.qf, nunits, P.grp, P.gflag, P.flist, P.tf, P.ngroups,
+                                        d->T.lorder, d->fbf);
   ECP_CHECK(hipGetLastError());
   double fb[2] = {0.0, 0.0};
   ECP_CHECK(hipMemcpyAsync(fb, d->fbf, sizeof(fb), hipMemcpyDeviceToHost, d->stream));

@@ -89,6 +89,9 @@ typedef struct {
   int t1_spec;        /* fused launches: the type-1 large-grid fallback starts after the head,
                          beside the tail, on every entry the head left open (1), or after the
                          tail on the failed ones only (0) */
+  int t2_spec;        /* fused launches with a type-2 head: the type-2 fallback starts after
+                         the head, beside the tail, on every entry the head left open (1), or
+                         after the tail on the failed ones only (0) */
 } ecp_tuning;
 
 /* ---- device layer (ecp_device.hip) ---- */

@@ -1,8 +1,8 @@
 """Launch parameters and the autotuner change how the type-1 small-grid
 integration is split between kernels (lane-serial head, wave-cooperative or
-lane-serial tail), whether the type-1 fallback starts beside the tail
-(speculatively) or after it, and how the other stages are launched, never
-their arithmetic: every setting must give the same matrix bit for bit."""
+lane-serial tail), whether the type-1 and type-2 fallbacks start beside
+their tails (speculatively) or after them, and how the other stages are
+launched, never their arithmetic: every setting must give the same matrix bit for bit."""
 import numpy as np
 import pytest
 
@@ -25,6 +25,8 @@ SETTINGS = [
     {"fuse": 0, "fb2_split": 4},
     {"fuse": 1, "t1_spec": 0},                                    # type-1 fallback after the tail
     {"fuse": 1, "t1_spec": 1, "t1_head_coop": 2},                 # ... beside it, from level 2 on
+    {"fuse": 1, "t2_spec": 0},                                    # type-2 fallback after the tail
+    {"fuse": 1, "t2_spec": 1, "t2_head": 3},                      # ... beside it, from level 3 on
 ]
 
 
