@@ -1920,7 +1920,10 @@ __device__ __forceinline__ double asm1_monomial(double acc, const double *__rest
 // synchronise together, an inactive group with empty loops.
 // MAXD: the largest monomial degree whose poly2sph row goes to registers
 // (the triplets' la + lb when a launch covers one class); higher degrees
-// take the general loop.
+// take the general loop.  Three waves per SIMD: the packed variant (four
+// triplets' state) then spills 52 bytes per lane, which costs less than
+// the fourth of its waves a spill-free budget would give up (the uneven
+// batch 0.8 ms slower).
 template <int G = 1, int MAXD = 6>
 __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, AsmArgs A,
                                                       unsigned char *__restrict__ status,
@@ -2725,7 +2728,9 @@ void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack,
     ASM1(0) ASM1(1) ASM1(2) ASM1(3) ASM1(4) ASM1(5)
 #undef ASM1
     default:
-      k_asm1<1, 6><<<g, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
+      // classes above 5 and mixed-class launches: rows of degree 6 take the
+      // general loop (holding them costs spills)
+      k_asm1<1, 5><<<g, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
   }
 }
 
