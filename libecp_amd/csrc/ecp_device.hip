@@ -1216,6 +1216,7 @@ struct T2Args {
   int *tclaim;                  // per triplet: queued for the fallback
   const unsigned char *tcls2;   // per triplet: its fallback class max(la, lb) + L - 1
   int spec;                     // the fallback takes every entry the head leaves open
+  unsigned char *fbf2;          // spec: per entry, the fallback failed on the large grid
 };
 
 // An entry for the large-grid fallback: gflag[g] counts the group's
@@ -1223,6 +1224,9 @@ struct T2Args {
 // queues the triplet, whose listed entries of all groups the fallback takes
 // together
 __device__ __forceinline__ void t2_enlist(const T2Args &A, int gid, int g, const T2Group &G) {
+  // the speculative fallback's failure flag of the entry, read only for
+  // enlisted entries: cleared here, before the fallback runs
+  if (A.spec) A.fbf2[gid] = 0;
   const int k = atomicAdd(&A.gflag[g], 1);
   A.flist[G.eoff + k] = gid - G.eoff;
   if (k == 0 && atomicExch(&A.tclaim[G.trip], 1) == 0) {
@@ -2656,7 +2660,7 @@ void launch_type2(ecp_dev *d, hipStream_t s, bool fused, bool spec = false, hipE
   const Prepared &P = d->P;
   if (P.ntot2 <= 0) return;
   T2Args A{P.grp, P.goff, P.ngroups, P.ntot2, d->tmpl_dev, P.Tout, P.tf, P.gflag, P.fbq2, P.fbc + kFb2Cnt, P.nt,
-           P.t2fb, P.flist, P.tq2, P.fbc + 48, P.tclaim, P.tcls2, spec ? 1 : 0};
+           P.t2fb, P.flist, P.tq2, P.fbc + 48, P.tclaim, P.tcls2, spec ? 1 : 0, P.fbf2};
   const int head = t2_head_levels(d, fused);
   {
     Span sp(d, s, FAM_TYPE2);
@@ -2728,9 +2732,10 @@ void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack,
     ASM1(0) ASM1(1) ASM1(2) ASM1(3) ASM1(4) ASM1(5)
 #undef ASM1
     default:
-      // classes above 5 and mixed-class launches: rows of degree 6 take the
-      // general loop (holding them costs spills)
-      k_asm1<1, 5><<<g, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
+      // classes above 5 and mixed-class launches: rows up to degree 6 in
+      // registers (84 bytes of spills per lane, against 0.7 ms more on the
+      // uneven batch's class-6 chain with degree 6 in the general loop)
+      k_asm1<1, 6><<<g, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
   }
 }
 
@@ -3262,8 +3267,8 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_t1fb = dv_take((size_t)nt);
   Ly.d_t2fb = dv_take((size_t)nt);
   Ly.d_tclaim = dv_take(4 * (size_t)nt);
-  Ly.d_fbf2 = dv_take((size_t)ntot2);
   Ly.d_zero_end = dv_off;
+  Ly.d_fbf2 = dv_take((size_t)ntot2);
   Ly.d_UL = dv_take(8 * job->nUL);
   Ly.d_Ut = dv_take(8 * job->nUt);
   Ly.d_F = dv_take(8 * job->nF);
