@@ -2815,11 +2815,11 @@ int fb2_split(const ecp_dev *d, bool fused) {
 }
 
 // The kernel sequence of one evaluation, on resident inputs, one launch per
-// class.  Tables run on the main stream.  The type-1 classes run on three
+// class.  Tables and the type-2 chain (small grid, fallbacks, assembly of the
+// triplets with fallback work) run on the main stream, the assembly of the
+// other type-2 triplets on a third worker, and the type-1 classes on two
 // forked streams, each class as small grid -> large-grid fallback ->
-// assembly.  Type 2 runs on the fourth: small grid, then the assembly of the
-// triplets without type-2 fallback work, while the main stream runs the
-// fallbacks and the assembly of the rest.  With evs, stage boundaries are
+// assembly.  With evs, stage boundaries are
 // recorded on the main stream: start, tables done, end, end (assembly
 // overlaps the quadratures).
 void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs);
@@ -2911,23 +2911,22 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (evs) (void)hipEventRecord(evs[1], s);
   (void)hipEventRecord(d->fork, s);
   for (int i = 0; i < kWorkers; i++) (void)hipStreamWaitEvent(w[i], d->fork, 0);
-  // type 2 on the last worker, then the assembly of the triplets without
-  // large-grid work there; the type-2 fallbacks and the assembly of the rest
-  // on the main stream, so that neither waits behind type-1 work
-  hipStream_t w2 = w[kWorkers - 1];
+  // four streams, one per hardware queue the device gives a process: the
+  // type-2 chain (small grid, fallbacks, assembly of the triplets with
+  // fallback work) on the main stream; the assembly of the other triplets
+  // on the third worker; the type-1 classes on the first two
   {
     Span sp(d, s, FAM_TABLES);
     launch_ang(d, s);
   }
   if (P.ntot2 > 0) {
-    launch_type2(d, w2, false);
-    (void)hipEventRecord(d->t2done, w2);
-    (void)hipStreamWaitEvent(w2, d->angdone, 0);
+    launch_type2(d, s, false);
+    (void)hipEventRecord(d->t2done, s);
+    (void)hipStreamWaitEvent(w[2], d->t2done, 0);
     {
-      Span sp(d, w2, FAM_ASM);
-      launch_asm2(d, w2, 1, false);
+      Span sp(d, w[2], FAM_ASM);
+      launch_asm2(d, w[2], 1, false);
     }
-    (void)hipStreamWaitEvent(s, d->t2done, 0);
     // every fallback class in one persistent grid: the classes' items run
     // side by side instead of one class after another on this stream
     {
@@ -2946,19 +2945,17 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       launch_asm2(d, s, 2, false);
     }
   }
-  // type-1 classes, each followed by its fallback and assembly on one of the
-  // other workers: the costliest class first, each on the worker with the
-  // least work so far (the cost of a class: its algorithmic small-grid
-  // FLOPs), so that no large class starts last and runs alone at the end
+  // type-1 classes, each followed by its fallback and assembly on one of two
+  // workers: the costliest class first, each on the worker with the least
+  // work so far (the cost of a class: its algorithmic small-grid FLOPs), so
+  // that no large class starts last and runs alone at the end
   int order[ECP_MAX_LAB + 1], nord = 0;
   for (int c = 0; c <= ECP_MAX_LAB; c++)
     if (P.cls_count[c] > 0) order[nord++] = c;
   std::stable_sort(order, order + nord, [&](int a, int b) { return P.flops_t1[a] > P.flops_t1[b]; });
-  double load[kWorkers - 1] = {};
+  double load[2] = {};
   for (int i = 0; i < nord; i++) {
-    int k = 0;
-    for (int j = 1; j < kWorkers - 1; j++)
-      if (load[j] < load[k]) k = j;
+    const int k = load[1] < load[0] ? 1 : 0;
     load[k] += P.flops_t1[order[i]];
     enqueue_t1_class(d, w[k], order[i]);
   }
