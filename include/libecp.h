@@ -36,7 +36,7 @@ typedef void (*ECPCallback)(int A, int sa, int la, int shifta,
    primitive dBS/aBS (used unnormalised).  ECP: shellsECP[atom] (0 = none),
    per ECP shell lECP/KECP, per primitive nECP (radial power including the
    r^2 Jacobian), dECP, aECP; the highest l of an atom is its local channel.
-   n: derivative order 0, 1 or 2 (basis l + n at most 5); for n > 0 each
+   n: derivative order 0, 1 or 2 (basis l + n at most 6); for n > 0 each
    triplet is delivered once per angular-momentum shift of the reference's
    table (libecp.c:243-250), with the shifts in the callback.
    lmax/shellOrdering: custom Cartesian ordering (NULL = libint).  Returns
