@@ -165,7 +165,7 @@ int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples);
    for small jobs), "asm2_cache" (type-2 assembly with the angular factors
    in registers: 0 no, 1 yes, 2, the default: with fused launches),
    "t2_head" (PS93 levels of the type-2 small-grid kernel before the entries
-   still open are rerun compacted; 15: no split; 0, the default: 6 with
+   still open are rerun compacted; 15, the default: no split; 0: 6 with
    fused launches, no split otherwise), "t2_tail_grid",
    "fb2_split" (waves sharing one (triplet, primitive pair) of the type-2
    large-grid fallback; 0, the default: 3 for jobs of up to a million

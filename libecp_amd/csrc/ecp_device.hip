@@ -2372,7 +2372,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 8, 100000, 8192, 2, 2, 0, 2048, 0, 1, 0};
+  ecp_tuning tune{8, 8, 100000, 8192, 2, 2, 15, 2048, 0, 1, 0};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph[kSpanGraphs] = {};
   hipGraphExec_t gexec[kSpanGraphs] = {};

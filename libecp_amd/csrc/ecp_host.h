@@ -82,7 +82,8 @@ typedef struct {
   int asm2_cache;     /* type-2 assembly with angular factors in registers: 0 no, 1 yes,
                          2 with fused launches only */
   int t2_head;        /* PS93 levels of the type-2 head kernel before the compacted tail
-                         (>= the level count: no tail; 0: 6 with fused launches, else none) */
+                         (>= the level count, the default: no tail; 0: 6 with fused
+                         launches, else none) */
   int t2_tail_grid;   /* most workgroups of the type-2 tail launch */
   int fb2_split;      /* waves per (triplet, primitive pair) of the type-2 fallback, its 64-entry
                          passes dealt round robin; 0: 3 for small jobs, 1 for large ones */

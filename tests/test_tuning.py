@@ -14,7 +14,7 @@ SETTINGS = [
     {"fuse": 1},                                                  # one launch per stage, all classes
     {"fuse": 1, "t1_head_coop": 2, "t1_tail_grid": 5},
     {"fuse": 1, "t1_head_coop": 15},                              # fused, no tail
-    {"fuse": 1, "asm2_cache": 0, "t2_head": 15},                 # type-2 without head/tail split
+    {"fuse": 1, "asm2_cache": 0, "t2_head": 6},                  # type-2 head/tail split at level 6
     {"fuse": 0, "asm2_cache": 1, "t2_head": 3, "t2_tail_grid": 2},
     {"fuse": 0, "t1_coop_max": 1 << 30, "t1_head_coop": 2},     # cooperative tail from level 2
     {"fuse": 0, "t1_coop_max": 1 << 30, "t1_head_coop": 12},
