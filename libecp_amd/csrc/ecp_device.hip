@@ -2815,11 +2815,9 @@ int fb2_split(const ecp_dev *d, bool fused) {
 }
 
 // The kernel sequence of one evaluation, on resident inputs, one launch per
-// class.  Tables and the type-2 chain (small grid, fallbacks, assembly of the
-// triplets with fallback work) run on the main stream, the assembly of the
-// other type-2 triplets on a third worker, and the type-1 classes on two
-// forked streams, each class as small grid -> large-grid fallback ->
-// assembly.  With evs, stage boundaries are
+// class: tables on the main stream, then the type-2 chain and the type-1
+// classes, each class as small grid -> large-grid fallback -> assembly, on
+// forked streams (two layouts, see enqueue).  With evs, stage boundaries are
 // recorded on the main stream: start, tables done, end, end (assembly
 // overlaps the quadratures).
 void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs);
@@ -2911,22 +2909,34 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   if (evs) (void)hipEventRecord(evs[1], s);
   (void)hipEventRecord(d->fork, s);
   for (int i = 0; i < kWorkers; i++) (void)hipStreamWaitEvent(w[i], d->fork, 0);
-  // four streams, one per hardware queue the device gives a process: the
+  // Two layouts.  Jobs whose type-1 classes stay at la + lb <= 6: four
+  // streams, one per hardware queue the device gives a process -- the
   // type-2 chain (small grid, fallbacks, assembly of the triplets with
-  // fallback work) on the main stream; the assembly of the other triplets
-  // on the third worker; the type-1 classes on the first two
+  // fallback work) on the main stream, the assembly of the other triplets
+  // on the third worker, the type-1 classes on the first two.  Jobs with
+  // larger classes, where type 1 carries more of the work: the type-2 small
+  // grid and the assembly without fallback work on the fourth worker, the
+  // fallbacks and the rest of the assembly on the main stream, the type-1
+  // classes on the first three workers.
+  const bool four = P.max_lab <= 6;
+  const int nw1 = four ? 2 : kWorkers - 1;
+  hipStream_t w2 = four ? s : w[kWorkers - 1];
+  hipStream_t wa = four ? w[2] : w2;  // the assembly without fallback work
   {
     Span sp(d, s, FAM_TABLES);
     launch_ang(d, s);
   }
   if (P.ntot2 > 0) {
-    launch_type2(d, s, false);
-    (void)hipEventRecord(d->t2done, s);
-    (void)hipStreamWaitEvent(w[2], d->t2done, 0);
+    launch_type2(d, w2, false);
+    (void)hipEventRecord(d->t2done, w2);
+    // the angular tables are on the main stream: behind the small grid in
+    // the four-stream layout, a wait otherwise
+    (void)hipStreamWaitEvent(wa, four ? d->t2done : d->angdone, 0);
     {
-      Span sp(d, w[2], FAM_ASM);
-      launch_asm2(d, w[2], 1, false);
+      Span sp(d, wa, FAM_ASM);
+      launch_asm2(d, wa, 1, false);
     }
+    if (!four) (void)hipStreamWaitEvent(s, d->t2done, 0);
     // every fallback class in one persistent grid: the classes' items run
     // side by side instead of one class after another on this stream
     {
@@ -2945,17 +2955,19 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       launch_asm2(d, s, 2, false);
     }
   }
-  // type-1 classes, each followed by its fallback and assembly on one of two
-  // workers: the costliest class first, each on the worker with the least
-  // work so far (the cost of a class: its algorithmic small-grid FLOPs), so
-  // that no large class starts last and runs alone at the end
+  // type-1 classes, each followed by its fallback and assembly on one of the
+  // type-1 workers: the costliest class first, each on the worker with the
+  // least work so far (the cost of a class: its algorithmic small-grid
+  // FLOPs), so that no large class starts last and runs alone at the end
   int order[ECP_MAX_LAB + 1], nord = 0;
   for (int c = 0; c <= ECP_MAX_LAB; c++)
     if (P.cls_count[c] > 0) order[nord++] = c;
   std::stable_sort(order, order + nord, [&](int a, int b) { return P.flops_t1[a] > P.flops_t1[b]; });
-  double load[2] = {};
+  double load[kWorkers - 1] = {};
   for (int i = 0; i < nord; i++) {
-    const int k = load[1] < load[0] ? 1 : 0;
+    int k = 0;
+    for (int j = 1; j < nw1; j++)
+      if (load[j] < load[k]) k = j;
     load[k] += P.flops_t1[order[i]];
     enqueue_t1_class(d, w[k], order[i]);
   }
