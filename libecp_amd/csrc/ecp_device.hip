@@ -1746,7 +1746,7 @@ struct AsmArgs {
   // and the type-2 region (blk, T) overlap
   int lds_blk, lds_H, lds_J, lds_rsph, lds_T, lds_rs_stride, pair_chunk, lds_Q;
   int lds_S1, lds_S2, shift_lm;  // shift factor tables (aliasing rsph/Q resp. T) for shells up to shift_lm
-  int lds_dir;                    // type 1: the pair directions of a chunk (4 values each)
+  const double *udir;             // type 1: per unit, cos theta, phi, sin phi, cos phi of its pair direction
   int lds_group;                  // packed type-1 assembly: LDS doubles per triplet
   // speculative type-1 fallback (qfb != nullptr): an entry whose small grid
   // failed (qfail bit) takes the fallback's result qfb; the triplet failed
@@ -1924,10 +1924,10 @@ __device__ __forceinline__ double asm1_monomial(double acc, const double *__rest
 // synchronise together, an inactive group with empty loops.
 // MAXD: the largest monomial degree whose poly2sph row goes to registers
 // (the triplets' la + lb when a launch covers one class); higher degrees
-// take the general loop.  Three waves per SIMD: the packed variant (four
-// triplets' state) then spills 52 bytes per lane, which costs less than
-// the fourth of its waves a spill-free budget would give up (the uneven
-// batch 0.8 ms slower).
+// take the general loop.  Registers, without spills: 51 (rows up to degree
+// 2 in registers) to 111 (degree 6), 94 for the packed variant; the pair
+// directions come from the tables kernel, so no trigonometric code sits
+// here.
 template <int G = 1, int MAXD = 6>
 __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, AsmArgs A,
                                                       unsigned char *__restrict__ status,
@@ -1942,7 +1942,6 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
   double *Jbuf = lds + A.lds_J;
   double *rsph = lds + A.lds_rsph;
   double *Qs = lds + A.lds_Q;
-  double *Dir = lds + A.lds_dir;
   int t;
   bool active = true;
   if constexpr (G == 1) {
@@ -1986,9 +1985,7 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
     }
   }
   const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
-  const double rAC[3] = {ra.x - rc.x, ra.y - rc.y, ra.z - rc.z};
-  const double rBC[3] = {rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
-  const V3 vA{rAC[0], rAC[1], rAC[2]}, vB{rBC[0], rBC[1], rBC[2]};
+  const V3 vA{ra.x - rc.x, ra.y - rc.y, ra.z - rc.z}, vB{rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
   const int D = T.ijkDim;
   const int tmd = ECP_DIM_L(T.tmDim);
 
@@ -2003,27 +2000,12 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
   for (int idx = lid; idx < np; idx += W) H[idx] = 0.0;
   for (int c0 = 0; c0 < npmax; c0 += A.pair_chunk) {
     const int nc = G == 1 ? min(A.pair_chunk, npairs - c0) : max(0, min(A.pair_chunk, npairs - c0));
-    // the chunk's pair directions (k of util.c:74-106 and its cos theta,
-    // sin phi, cos phi, as spherical_harmonics.c:15-114 forms them), one
-    // lane per pair, then their harmonics, one (pair, m) column per lane;
-    // the pairs' Q values staged alongside
-    for (int k = lid; k < nc; k += W) {
-      const T1Unit U = A.units1[u0 + c0 + k];
-      const double za = J.pa[Sa.prim + U.pa], zb = J.pa[Sb.prim + U.pb];
-      double P[3];
-      for (int p = 0; p < 3; p++) P[p] = 2.0 * (za * rAC[p] + zb * rBC[p]);
-      double r, th, ph;
-      sph_coords(P[0], P[1], P[2], r, th, ph);
-      double *d = Dir + 4 * k;
-      d[0] = cos(th);
-      d[1] = ph;
-      d[2] = sin(ph);
-      d[3] = cos(ph);
-    }
-    __syncthreads();
+    // the harmonics of the chunk's pair directions (formed by the tables
+    // kernel), one (pair, m) column per lane; the pairs' Q values staged
+    // alongside
     for (int it = lid; it < nc * (lab + 1); it += W) {
       const int k = it / (lab + 1), m = it - k * (lab + 1);
-      const double *d = Dir + 4 * k;
+      const double *d = A.udir + 4 * (size_t)(u0 + c0 + k);
       real_sph_column(lab, m, d[0], d[1], d[2], d[3], A.fac, A.dfac, rsph + k * A.lds_rs_stride);
     }
     for (int it = lid; it < nc * nql; it += W) {
@@ -2231,14 +2213,38 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 // The per-job tables in one launch of 128-thread blocks: the zero-fill of
 // per-run counters and flags (a kernel, so that the sequence can be captured
 // in a HIP graph without memset nodes), the U tables per (centre, row, point
-// chunk) and the F slabs per (centre, shell, run).  They are independent of
-// each other; every consumer is a later launch.
+// chunk), the F slabs per (centre, shell, run) and the pair directions of
+// the type-1 units.  They are independent of each other; every consumer is
+// a later launch.
 struct TabArgs {
   unsigned int *zero_base;
   size_t zero_words;
   int nzb, nub, nfit, rows;  // rows: U-table rows per centre (largest L + 1)
   const int2 *fit;
+  const T1Unit *units;       // type-1 units and their pair directions, 128 per workgroup
+  int nunits;
+  double *udir;
 };
+
+// The pair direction k = 2 (za rAC + zb rBC) of a type-1 unit (type1.c:
+// 211-302 via util.c:74-106), stored as cos theta, phi, sin phi, cos phi,
+// the values spherical_harmonics.c:15-114 forms from it.  Formed with the
+// tables, so that the type-1 assembly carries no trigonometric code (with
+// it, 168 registers and spills instead of 111).
+__device__ void unit_dir(const DevJob &J, const T1Unit &U, double *out) {
+  const ecp_triplet tr = J.trip[U.trip];
+  const ecp_centre Cn = J.cent[tr.centre];
+  const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
+  const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
+  const double za = J.pa[Sa.prim + U.pa], zb = J.pa[Sb.prim + U.pb];
+  double r, th, ph;
+  sph_coords(2.0 * (za * (ra.x - rc.x) + zb * (rb.x - rc.x)), 2.0 * (za * (ra.y - rc.y) + zb * (rb.y - rc.y)),
+             2.0 * (za * (ra.z - rc.z) + zb * (rb.z - rc.z)), r, th, ph);
+  out[0] = cos(th);
+  out[1] = ph;
+  out[2] = sin(ph);
+  out[3] = cos(ph);
+}
 
 constexpr int kUtabChunks = (SO + 127) / 128;
 
@@ -2256,7 +2262,13 @@ __global__ __launch_bounds__(128, 4) void k_tables(DevTables T, DevJob J, TabArg
     return;
   }
   b -= A.nub;
-  if (b < A.nfit) ftab_item(T, J, A.fit[b]);
+  if (b < A.nfit) {
+    ftab_item(T, J, A.fit[b]);
+    return;
+  }
+  b -= A.nfit;
+  const int u = b * 128 + threadIdx.x;
+  if (u < A.nunits) unit_dir(J, A.units[u], A.udir + 4 * (size_t)u);
 }
 
 // angular tables: a launch of their own (few registers, high occupancy),
@@ -2274,7 +2286,7 @@ struct JobLayout {
          i_u1 = 0, i_t1f = 0, i_grp = 0, i_goff = 0, i_tgf = 0, i_fit = 0, i_ait = 0, i_tcls = 0, i_tcls2 = 0,
          in_bytes = 0;
   size_t d_zero = 0, d_fbc = 0, d_gflag = 0, d_stat = 0, d_t1fb = 0, d_t2fb = 0, d_tclaim = 0, d_fbf2 = 0, d_zero_end = 0, d_UL = 0, d_Ut = 0, d_F = 0, d_ang = 0,
-         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_qfb = 0, d_qopen = 0, d_qfbf = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
+         d_q = 0, d_t1s = 0, d_tq1 = 0, d_qf = 0, d_qfb = 0, d_qopen = 0, d_qfbf = 0, d_udir = 0, d_fbq1 = 0, d_T = 0, d_tf = 0, d_flist = 0, d_tq2 = 0, d_fbq2 = 0, d_fbT = 0, d_blk = 0, total = 0;
   int FL = 0, UK = 0, angL = 0, ang_lam = 0, ang_c = 0, deriv = 0;
 };
 
@@ -2297,6 +2309,7 @@ struct Prepared {
   unsigned long long *qf = nullptr;
   unsigned long long *qopen = nullptr, *qfbf = nullptr;  // speculative type-1 fallback
   double *qfb = nullptr;
+  double *udir = nullptr;  // per type-1 unit: its pair direction (4 values)
   int *fbq1 = nullptr, *fbc = nullptr;
   unsigned char *stat = nullptr;
   unsigned char *t1fb = nullptr, *t2fb = nullptr;  // per triplet: has large-grid work (type 1 / type 2)
@@ -2733,8 +2746,8 @@ void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack,
 #undef ASM1
     default:
       // classes above 5 and mixed-class launches: rows up to degree 6 in
-      // registers (84 bytes of spills per lane, against 0.7 ms more on the
-      // uneven batch's class-6 chain with degree 6 in the general loop)
+      // registers (the uneven batch's class-6 chain 0.7 ms slower with
+      // degree 6 in the general loop)
       k_asm1<1, 6><<<g, kAsmThreads, P.asm1_lds, s>>>(d->T, P.J, aa, P.stat, P.tcls + m, e - m, P.t1fb, filter);
   }
 }
@@ -2832,7 +2845,10 @@ void launch_tables(ecp_dev *d, hipStream_t s) {
   A.nub = P.ncent * A.rows * kUtabChunks;
   A.nfit = P.nfit;
   A.fit = P.fit;
-  k_tables<<<A.nzb + A.nub + A.nfit, 128, 0, s>>>(d->T, P.J, A);
+  A.units = P.u1;
+  A.nunits = P.cls_begin[ECP_MAX_LAB + 1];
+  A.udir = P.udir;
+  k_tables<<<A.nzb + A.nub + A.nfit + (A.nunits + 127) / 128, 128, 0, s>>>(d->T, P.J, A);
 }
 
 // the angular tables on stream s, concurrent with the quadratures; the
@@ -3298,6 +3314,7 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
   Ly.d_qfb = dv_take(8 * nq);
   Ly.d_qopen = dv_take(8 * units1.size());
   Ly.d_qfbf = dv_take(8 * units1.size());
+  Ly.d_udir = dv_take(32 * units1.size());
   Ly.total = dv_off;
   Ly.FL = job->FL;
   Ly.UK = job->UK;
@@ -3324,7 +3341,6 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
     aa.lds_S1 = o;
     aa.lds_rsph = o; o += aa.pair_chunk * aa.lds_rs_stride;
     aa.lds_Q = o; o += aa.pair_chunk * nq_of(2 * LM);
-    aa.lds_dir = o; o += 4 * aa.pair_chunk;
     const int end1 = std::max(o, aa.lds_S1 + ns);
     // type 2: blk stays for the shift; the shift tables reuse T
     o = base;
@@ -3346,7 +3362,6 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
       a4.lds_S1 = o4;
       a4.lds_rsph = o4; o4 += a4.pair_chunk * a4.lds_rs_stride;
       a4.lds_Q = o4; o4 += a4.pair_chunk * nq_of(LQ);
-      a4.lds_dir = o4; o4 += 4 * a4.pair_chunk;
       a4.lds_group = std::max(o4, a4.lds_S1 + shift_tab_doubles(LS));
       P.asm1_lds4 = (size_t)kAsmPack * a4.lds_group * sizeof(double);
     }
@@ -3462,6 +3477,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.qfb = (double *)(A + Ly.d_qfb);
   P.qopen = (unsigned long long *)(A + Ly.d_qopen);
   P.qfbf = (unsigned long long *)(A + Ly.d_qfbf);
+  P.udir = (double *)(A + Ly.d_udir);
   P.fbq1 = (int *)(A + Ly.d_fbq1);
   P.fbc = (int *)(A + Ly.d_fbc);  // [0..12] type-1 fallback queues, [32..44] type-1 tails, [48] type-2
                                   // tail, [kFb2Cnt + 0..12] type-2 fallback queues
@@ -3490,6 +3506,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   for (AsmArgs *x : {&P.aa, &P.aa4}) {
     AsmArgs &aa = *x;
     aa.units1 = P.u1;
+    aa.udir = P.udir;
     aa.t1first = (const int *)(A + Ly.i_t1f);
     aa.qout = P.q;
     aa.groups = P.grp;
