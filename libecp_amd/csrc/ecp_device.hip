@@ -909,8 +909,11 @@ __device__ int fm06_break(const DevTables &T, const Fm06 &M, double dAC, double 
 // levels.  Summation order inside each level is unchanged.
 __device__ __forceinline__ int psm92_points(int lv) { return lv == 0 ? 1 : (1 << lv); }
 
-__device__ __forceinline__ int psm92_round_end(int lv, int nlev) {
-  return min(nlev, lv == 0 ? 2 : (lv == 2 ? 6 : lv + 1));
+// hw: the slots of a round (lanes per integration, 16, 32 or 64); levels
+// 2 .. l2 - 1 take 2^l2 - 4 <= hw of them
+__device__ __forceinline__ int psm92_round_end(int lv, int nlev, int hw = 64) {
+  const int l2 = hw >= 64 ? 6 : hw >= 32 ? 5 : 4;
+  return min(nlev, lv == 0 ? 2 : (lv == 2 ? l2 : lv + 1));
 }
 
 // grid index evaluated by `slot` in the round [l0, l1); -1 for none (a slot
@@ -1356,13 +1359,23 @@ __global__ __launch_bounds__(64) void k_type2_tail(DevTables T, DevJob J, T2Args
 // for all failed entries of all the triplet's channels; lanes own the failed
 // entries (64 per pass) and sum the points in the integrator's pair order.
 // Results per (entry, pair) go to fbT and are summed over pairs, in pair
-// order, by the assembly.
+// order, by the assembly.// type-2 fallback (calcT_FM06, type2.c:417-528): one wave per (triplet with
+// failed T entries, primitive pair).  The FM06-mapped points depend on the
+// pair only, so the wave evaluates each level's points once (K_a, K_b, the
+// exponential, and U_l for every channel l with failed entries) into LDS
+// for all failed entries of all the triplet's channels; lanes own the
+// failed entries (64 per pass) and sum the points in the integrator's pair
+// With PACK, a triplet with at most 16 (32) failed entries runs four (two)
+// of its pairs per wave instead, one per lane group of HW = 16 (32) lanes,
+// so that the lanes summing are not a few of 64: the item of every fourth
+// (second) pair takes them, the others none; the groups share the wave's
+// rounds, each over its own pair's points.
 constexpr int kFb2PerLane = 1;
 constexpr int kFb2Cnt = 52;  // fbc[kFb2Cnt + c]: triplets queued for the type-2 fallback, class c
 constexpr int kFbcSlots = 80;  // queue counters per job
 static_assert(ECP_MAX_LAB < 16 && kFb2Cnt + kMaxLMB < kFbcSlots, "queue counter slots");
 
-template <int LMB>
+template <int LMB, bool PACK>
 __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, const DevJob &J,
                                          const T2Group *__restrict__ groups, const int *__restrict__ tmpl,
                                          const unsigned char *__restrict__ tfail, const int *__restrict__ fbq,
@@ -1384,8 +1397,8 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     const int pair = (int)(r % J.maxPairs);
     const ecp_triplet tr = J.trip[t];
     const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
-    if (pair >= Sa.K * Sb.K) continue;
-    const int pa = pair / Sb.K, pb = pair % Sb.K;
+    const int npairs = Sa.K * Sb.K;
+    if (pair >= npairs) continue;
     const ecp_centre Cn = J.cent[tr.centre];
     // the triplet's groups (one per semi-local channel l) and their failed
     // entries; the points are shared by all of them
@@ -1398,6 +1411,16 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
     }
     // this wave's passes: split, split + nsplit, ... (none: nothing to do)
     if (split * 64 * kFb2PerLane >= nfail) continue;
+    // lane groups of HW = 1 << hwl lanes, group g on pair + g (a group past
+    // the triplet's pairs idles on pair's points)
+    const int hwl = !PACK ? 6 : nfail <= 16 ? 4 : nfail <= 32 ? 5 : 6;
+    if (PACK && (pair & ((64 >> hwl) - 1))) continue;
+    const int HW = PACK ? 1 << hwl : 64, NG = PACK ? 64 >> hwl : 1;
+    const int grp = PACK ? lane >> hwl : 0, sl = PACK ? lane & (HW - 1) : lane, base = PACK ? grp << hwl : 0;
+    const int mypair = pair + grp;
+    const bool gvalid = !PACK || mypair < npairs;
+    const int pq = gvalid ? mypair : pair;
+    const int pa = pq / Sb.K, pb = pq % Sb.K;
     const ecp_xyz rc = J.atoms[Cn.atom];
     const double dAC = dist(rc, J.atoms[Sa.atom]), dBC = dist(rc, J.atoms[Sb.atom]);
     const double za = J.pa[Sa.prim + pa], zb = J.pa[Sb.prim + pb];
@@ -1413,7 +1436,24 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
       const double d1 = dAC - r, d2 = dBC - r;
       return -za * d1 * d1 - zb * d2 * d2;
     };
-    const int brk = fm06_break(T, M, dAC, dBC, minExp, expo_of);
+    // the search is wave-wide: with lane groups on different pairs, the
+    // whole wave finds each group's break in turn
+    int brk = 0;
+    if (NG == 1) {
+      brk = fm06_break(T, M, dAC, dBC, minExp, expo_of);
+    } else {
+      for (int q = 0; q < NG; q++) {
+        const int pq2 = pair + q < npairs ? pair + q : pair;
+        const double za2 = J.pa[Sa.prim + pq2 / Sb.K], zb2 = J.pa[Sb.prim + pq2 % Sb.K];
+        const double zp2 = za2 + zb2;
+        const Fm06 M2 = fm06(zp2, (za2 * dAC + zb2 * dBC) / zp2);
+        const int b2 = fm06_break(T, M2, dAC, dBC, minExp, [&](double r) {
+          const double d1 = dAC - r, d2 = dBC - r;
+          return -za2 * d1 * d1 - zb2 * d2 * d2;
+        });
+        if (q == grp) brk = b2;
+      }
+    }
     const int end = brk - 1;
     const int laC = tr.la + lmax, lbC = tr.lb + lmax;
     const ecp_gauss *gg = J.gauss + Cn.g0;
@@ -1458,21 +1498,21 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
       }
     };
     // the failed entries of all groups (listed by k_type2), group after
-    // group, 64 per pass
-    for (int pass0 = split * 64 * kFb2PerLane; pass0 < nfail; pass0 += 64 * kFb2PerLane * nsplit) {
-      const int nhere = min(64 * kFb2PerLane, nfail - pass0);
+    // group, HW per pass
+    for (int pass0 = split * HW * kFb2PerLane; pass0 < nfail; pass0 += HW * kFb2PerLane * nsplit) {
+      const int nhere = min(HW * kFb2PerLane, nfail - pass0);
       int ent[kFb2PerLane], c1[kFb2PerLane], c2[kFb2PerLane], c3[kFb2PerLane], el[kFb2PerLane];
       int eo[kFb2PerLane];  // the entry's index in the job's T arrays
       double I[kFb2PerLane], p[kFb2PerLane], res[kFb2PerLane];
       unsigned int done = 0;
 #pragma unroll
       for (int j = 0; j < kFb2PerLane; j++) {
-        const int k = lane + 64 * j;
+        const int k = sl + HW * j;
         ent[j] = -1;
         el[j] = 0;
         eo[j] = 0;
         int code = 0;
-        if (k < nhere) {
+        if (gvalid && k < nhere) {
           int kk = pass0 + k, l = 0;
           while (kk >= gc[l]) kk -= gc[l++];
           const T2Group G = groups[g0 + l];
@@ -1535,16 +1575,16 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
       };
       int lv = 0;
       while (lv < T.s92.nlevels && wave_any(done != all)) {
-        const int l1 = psm92_round_end(lv, T.s92.nlevels);
-        if (l1 > lv + 1 || psm92_points(lv) <= 64) {
-          eval(lane, psm92_round_point(T, lv, l1, lane, end));
+        const int l1 = psm92_round_end(lv, T.s92.nlevels, HW);
+        if (l1 > lv + 1 || psm92_points(lv) <= HW) {
+          eval(base + sl, psm92_round_point(T, lv, l1, sl, end));
           __syncthreads();
           int roff = 0;
           for (int l = lv; l < l1; l++) {
             if (l == 0) {
 #pragma unroll
               for (int j = 0; j < kFb2PerLane; j++)
-                if (!(done & (1u << j))) I[j] = p[j] = f(j, 0);
+                if (!(done & (1u << j))) I[j] = p[j] = f(j, base);
             } else {
               double q[kFb2PerLane];
 #pragma unroll
@@ -1552,7 +1592,7 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
                 q[j] = 2 * p[j];
                 p[j] = 2 * I[j];
               }
-              sum(l, roff, 0, T.s92.lvl_begin[l + 1] - T.s92.lvl_begin[l]);
+              sum(l, base + roff, 0, T.s92.lvl_begin[l + 1] - T.s92.lvl_begin[l]);
               test(l, q);
             }
             roff += psm92_points(l);
@@ -1567,17 +1607,17 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
             p[j] = 2 * I[j];
           }
           const int b0 = T.s92.lvl_begin[lv], b1 = T.s92.lvl_begin[lv + 1];
-          for (int cb = b0; cb < b1; cb += 32) {
-            const int nc = min(32, b1 - cb);
-            const int k = lane >> 1;
+          for (int cb = b0; cb < b1; cb += HW / 2) {
+            const int nc = min(HW / 2, b1 - cb);
+            const int k = sl >> 1;
             int n = -1;
             if (k < nc) {
-              if ((lane & 1) == 0) n = T.p92l[cb + k];
+              if ((sl & 1) == 0) n = T.p92l[cb + k];
               else if (T.p92r[cb + k] <= end) n = T.p92r[cb + k];
             }
-            eval(lane, n);
+            eval(base + sl, n);
             __syncthreads();
-            sum(lv, 0, cb - b0, cb - b0 + nc);
+            sum(lv, base, cb - b0, cb - b0 + nc);
             __syncthreads();
           }
           test(lv, q);
@@ -1587,7 +1627,7 @@ __device__ __forceinline__ void fb2_body(int bid, int nb, const DevTables &T, co
 #pragma unroll
       for (int j = 0; j < kFb2PerLane; j++) {
         if (ent[j] < 0) continue;
-        if (done & (1u << j)) fbT[(size_t)eo[j] * J.maxPairs + pair] = res[j];
+        if (done & (1u << j)) fbT[(size_t)eo[j] * J.maxPairs + mypair] = res[j];
         else if (fbf2) fbf2[eo[j]] = 1;
         else status[2 * t + 1] = 1;
       }
@@ -1704,8 +1744,10 @@ struct T2Set {
   unsigned char *fbf2;  // speculative fallback: per entry, the large grid failed (nullptr: off)
 };
 
-template <int MAXLMB>
-__global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S, ClassMap cm) {
+// PACK: triplets with few failed entries run several pairs per wave (three
+// waves per SIMD kept: at two the uneven batch loses the gain)
+template <int MAXLMB, bool PACK>
+__global__ __launch_bounds__(64, PACK ? 3 : 1) void k_fb2_fused(DevTables T, DevJob J, T2Set S, ClassMap cm) {
   __shared__ double s_r[64], s_w[64], s_cu[ECP_MAX_L * 64], s_ex[64];
   __shared__ double s_ka[(MAXLMB + 1) * kKS], s_kb[(MAXLMB + 1) * kKS];
   const int i = class_entry(cm, blockIdx.x);
@@ -1714,7 +1756,7 @@ __global__ __launch_bounds__(64) void k_fb2_fused(DevTables T, DevJob J, T2Set S
 #define X(L)                                                                                                   \
   case L:                                                                                                      \
     if constexpr (L <= MAXLMB)                                                                                 \
-      fb2_body<L>(bid, nb, T, J, S.groups, S.tmpl, S.tf, S.fbq2 + (size_t)L * S.nt, S.fbc + kFb2Cnt + L, S.fbT, \
+      fb2_body<L, PACK>(bid, nb, T, J, S.groups, S.tmpl, S.tf, S.fbq2 + (size_t)L * S.nt, S.fbc + kFb2Cnt + L, S.fbT, \
                   S.stat, S.gcount, S.flist, S.tgfirst, S.nsplit, s_r, s_w, s_cu, s_ex, s_ka, s_kb, S.fbf2);    \
     break;
     ECP_CASES_LMB(X)
@@ -2803,14 +2845,19 @@ void launch_fb1(ecp_dev *d, hipStream_t s, const DevTables &T, const DevJob &J, 
 }
 
 // the type-2 fallback classes of cm in one grid; like the type-1 kernels,
-// three instantiations bound the Bessel orders the grid holds: up to 7, 11
-// and 12
-void launch_fb2_fused(ecp_dev *d, hipStream_t s, const T2Set &S2, const ClassMap &cm) {
+// three tiers bound the Bessel orders the grid holds: up to 7, 11 and 12
+// pack: several pairs per wave for triplets with few failed entries
+// (classes up to 7).  Less work in all, but each wave longer: it pays in
+// the throughput-bound per-class sequence of large jobs (the uneven batch
+// 1.4 % faster) and not on the fused sequence's critical chain (the medium
+// cluster 1.5 % slower).
+void launch_fb2_fused(ecp_dev *d, hipStream_t s, const T2Set &S2, const ClassMap &cm, bool pack) {
   const Prepared &P = d->P;
   const int nb = cm.bbeg[cm.n];
-  if (P.max_cls2 > 11) k_fb2_fused<12><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
-  else if (P.max_cls2 > 7) k_fb2_fused<11><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
-  else k_fb2_fused<7><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
+  if (P.max_cls2 > 11) k_fb2_fused<12, false><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
+  else if (P.max_cls2 > 7) k_fb2_fused<11, false><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
+  else if (pack) k_fb2_fused<7, true><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
+  else k_fb2_fused<7, false><<<nb, 64, 0, s>>>(d->T, P.J, S2, cm);
 }
 
 // waves per (triplet, pair) of the type-2 fallback (fb2_split; 0: automatic,
@@ -2963,7 +3010,7 @@ void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
       });
       if (cm2.n > 0) {
         Span sp(d, s, FAM_FB2);
-        launch_fb2_fused(d, s, S2, cm2);
+        launch_fb2_fused(d, s, S2, cm2, true);
       }
     }
     {
@@ -3126,7 +3173,7 @@ void enqueue_fused(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
     });
     if (cm2.n > 0) {
       Span sp(d, w[3], FAM_FB2);
-      launch_fb2_fused(d, w[3], S2, cm2);
+      launch_fb2_fused(d, w[3], S2, cm2, false);
     }
     (void)hipStreamWaitEvent(w[2], d->angdone, 0);
     {
