@@ -2110,15 +2110,18 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
 // A per lambda1; NB = 0: both read for each (lambda1, lambda2), with NM > 1
 // the 2 l + 1 values of each loaded together (NM >= 2 l + 1 of every
 // projector l).
+// tlist: the triplets of the launch, one per workgroup (nullptr: workgroup
+// t takes triplet t).
 template <int NB, int NM, int TPB = kAsmThreads>
 __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
                                                       unsigned char *__restrict__ status,
-                                                      const unsigned char *__restrict__ fbflag, int filter) {
+                                                      const unsigned char *__restrict__ fbflag, int filter,
+                                                      const int *__restrict__ tlist) {
   extern __shared__ double lds[];
   double *blk = lds + A.lds_blk;
   double *Jbuf = lds + A.lds_J;
   double *Tl = lds + A.lds_T;
-  const int t = blockIdx.x;
+  const int t = tlist ? tlist[blockIdx.x] : (int)blockIdx.x;
   if ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t])) return;
   if (status[2 * t + 1]) return;
   const ecp_triplet tr = J.trip[t];
@@ -2239,6 +2242,127 @@ __global__ __launch_bounds__(TPB) void k_asm2(DevTables T, DevJob J, AsmArgs A,
   const double norm2 = (4.0 * M_PI) * (4.0 * M_PI);
   auto get2 = [&](int r, int q) { return blk[r * ncb + q]; };
   shift_block(T, get2, tr.oa, tr.ob, vA, vB, norm2, Jbuf, lds + A.lds_S2, A.shift_lm, A.blocks + tr.blk2);
+}
+
+// Type-2 blocks of classes la + lb <= kAsmPackLab (at most 9 elements),
+// kAsmPack triplets per wave (tlist[0 .. ntl)): lane groups of
+// kAsmThreads / kAsmPack lanes, each with an LDS slice of A.lds_group
+// doubles, run k_asm2's phases side by side with the same arithmetic (NM as
+// there: the 2 l + 1 angular factors loaded together, or dot_in_order for
+// NM = 1).  The channel loop runs to the wave's largest L, a group past its
+// own with empty loops; every group synchronises with the wave.
+template <int NM>
+__global__ __launch_bounds__(kAsmThreads) void k_asm2p(DevTables T, DevJob J, AsmArgs A,
+                                                    unsigned char *__restrict__ status,
+                                                    const int *__restrict__ tlist, int ntl,
+                                                    const unsigned char *__restrict__ fbflag, int filter) {
+  extern __shared__ double lds_all[];
+  constexpr int G = kAsmPack, W = kAsmThreads / G;
+  const int grp = (int)threadIdx.x / W, lid = (int)threadIdx.x % W;
+  double *lds = lds_all + (size_t)grp * A.lds_group;
+  double *blk = lds + A.lds_blk;
+  double *Jbuf = lds + A.lds_J;
+  double *Tl = lds + A.lds_T;
+  const int ti = blockIdx.x * G + grp;
+  bool active = ti < ntl;
+  const int t = active ? tlist[ti] : tlist[0];
+  if (active && ((filter == 1 && fbflag[t]) || (filter == 2 && !fbflag[t]) || status[2 * t + 1])) active = false;
+  if (!wave_any(active)) return;
+  const ecp_triplet tr = J.trip[t];
+  const ecp_centre Cn = J.cent[tr.centre];
+  const ecp_shell Sa = J.shells[tr.sa], Sb = J.shells[tr.sb];
+  const int la = tr.la, lb = tr.lb, lab = la + lb;  // shifted for derivatives
+  const int nca = ECP_DIM_C(la), ncb = ECP_DIM_C(lb);
+  const ecp_xyz rc = J.atoms[Cn.atom], ra = J.atoms[Sa.atom], rb = J.atoms[Sb.atom];
+  const V3 vA{ra.x - rc.x, ra.y - rc.y, ra.z - rc.z}, vB{rb.x - rc.x, rb.y - rc.y, rb.z - rc.z};
+  const int g0 = A.tgfirst[t];
+  if (A.fbf2 && filter == 2) {
+    // speculative fallback: the triplet failed where an entry failed on the
+    // small grid and on the large grid (type2.c:417-528, code 1); the
+    // verdict of the group's lanes
+    bool bad = false;
+    if (active)
+      for (int l = 0; l < Cn.L; l++) {
+        const T2Group Gr = A.groups[g0 + l];
+        for (int e = lid; e < Gr.ne; e += W) bad = bad || (A.tfail[Gr.eoff + e] && A.fbf2[Gr.eoff + e]);
+      }
+    const unsigned long long m = __ballot(bad);
+    if ((m >> (grp * W)) & ((1ull << W) - 1ull)) {
+      if (lid == 0) status[2 * t + 1] = 1;
+      active = false;
+    }
+    if (!wave_any(active)) return;
+  }
+  if (active)
+    for (int i = lid; i < nca * ncb; i += W) blk[i] = 0.0;
+  int Lw = active ? Cn.L : 0;
+  for (int o = 32; o > 0; o >>= 1) Lw = max(Lw, __shfl_xor(Lw, o, 64));
+  __syncthreads();
+  const int nlm2 = J.angL * J.angL;
+  const double *angA = J.ang + Cn.ang_off + (size_t)(Sa.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
+  const double *angB = J.ang + Cn.ang_off + (size_t)(Sb.atom - Cn.atom0) * J.ang_lam * nlm2 * J.ang_c;
+  for (int l = 0; l < Lw; l++) {
+    const bool on = active && l < Cn.L;
+    const int d1 = la + l + 1, d2 = lb + l + 1, d3 = lab + 1;
+    T2Group Gr{};
+    if (on) {
+      Gr = A.groups[g0 + l];
+      for (int i = lid; i < d1 * d2 * d3; i += W) Tl[i] = 0.0;
+    }
+    __syncthreads();
+    const int nm = 2 * l + 1;
+    if (on)
+      for (int e = lid; e < Gr.ne; e += W) {
+        const int code = A.tmpl[Gr.tmpl + e];
+        const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
+        double v;
+        if (!A.tfail[Gr.eoff + e]) {
+          v = A.Tout[Gr.eoff + e];
+        } else {
+          v = sum_in_order(0.0, A.fbT + (size_t)(Gr.eoff + e) * J.maxPairs, Sa.K * Sb.K);
+        }
+        Tl[(l1 * d2 + l2) * d3 + l3] = v;
+      }
+    __syncthreads();
+    if (on)
+      for (int idx = lid; idx < nca * ncb; idx += W) {
+        const int pp = idx / ncb, qq = idx % ncb;
+        int alpha = 0, beta = 0;
+        while (ECP_DIM_C(alpha) <= pp) alpha++;
+        while (ECP_DIM_C(beta) <= qq) beta++;
+        const int par1 = (alpha + l) % 2, par2 = (beta + l) % 2;
+        const int ll1 = (par1 > l - alpha) ? par1 : l - alpha;
+        const int ll2 = (par2 > l - beta) ? par2 : l - beta;
+        double tmp = 0.0;
+        for (int lam1 = ll1; lam1 <= la + l; lam1 += 2)
+          for (int lam2 = ll2; lam2 <= lb + l; lam2 += 2) {
+            const double *pa = angA + ((size_t)lam1 * nlm2 + l * l) * J.ang_c + pp;
+            const double *pb = angB + ((size_t)lam2 * nlm2 + l * l) * J.ang_c + qq;
+            double factor;
+            if constexpr (NM > 1) {
+              double x[NM], y[NM];
+#pragma unroll
+              for (int m = 0; m < NM; m++) {
+                x[m] = m < nm ? pa[m * J.ang_c] : 0.0;
+                y[m] = m < nm ? pb[m * J.ang_c] : 0.0;
+              }
+              factor = 0.0;
+#pragma unroll
+              for (int m = 0; m < NM; m++)
+                if (m < nm) factor += x[m] * y[m];
+            } else {
+              factor = dot_in_order<2>(0.0, pa, J.ang_c, pb, J.ang_c, nm);
+            }
+            tmp += factor * Tl[(lam1 * d2 + lam2) * d3 + alpha + beta];
+          }
+        blk[idx] += tmp;
+      }
+    __syncthreads();
+  }
+  const double norm2 = (4.0 * M_PI) * (4.0 * M_PI);
+  auto get2 = [&](int r, int q) { return blk[r * ncb + q]; };
+  shift_block(T, get2, tr.oa, tr.ob, vA, vB, norm2, Jbuf, lds + A.lds_S2, A.shift_lm, A.blocks + tr.blk2, lid, W,
+              active);
 }
 
 // ---------------------------------------------------------------------------
@@ -2378,7 +2502,8 @@ struct Prepared {
   unsigned int *zero_base = nullptr;
   size_t zero_words = 0;
   double *blk = nullptr;
-  size_t asm1_lds = 0, asm2_lds = 0, asm1_lds4 = 0;
+  size_t asm1_lds = 0, asm2_lds = 0, asm1_lds4 = 0, asm2_lds4 = 0;
+  AsmArgs aa2p{};  // packed type-2 assembly of classes la + lb <= kAsmPackLab
   int tcls_begin[ECP_MAX_LAB + 2] = {0};  // triplets of each class lambda = la + lb in tcls
   const int *tcls = nullptr;
   // algorithmic FLOPs per evaluation (the per-unit models of the reference's
@@ -2733,21 +2858,37 @@ void launch_type2(ecp_dev *d, hipStream_t s, bool fused, bool spec = false, hipE
 // l <= 4 with projectors l <= 4; larger jobs read the tables directly.  The
 // cache costs occupancy: it pays where the assembly is on the critical path
 // of a small job (fused launches), not in large throughput-bound ones.
+// pack: the triplets of classes la + lb <= kAsmPackLab (the first of the
+// class-ordered list) kAsmPack to a wave, the others one per workgroup
 template <int TPB>
-void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache, bool spec) {
+void launch_asm2_tpb(ecp_dev *d, hipStream_t s, int filter, bool cache, bool spec, bool pack) {
   const Prepared &P = d->P;
   AsmArgs aa = P.aa;
   aa.fbf2 = spec ? P.fbf2 : nullptr;
+  const int m = pack ? P.tcls_begin[kAsmPackLab + 1] : 0;
+  if (m > 0) {
+    AsmArgs a2 = P.aa2p;
+    a2.fbf2 = aa.fbf2;
+    const int g = (m + kAsmPack - 1) / kAsmPack;
+    // the 2 l + 1 angular factors loaded together where l < 4 (148
+    // registers, three waves per SIMD, against 84 and five read pair by
+    // pair: the uneven batch 0.7 ms faster)
+    if (P.max_L <= 4) k_asm2p<7><<<g, kAsmThreads, P.asm2_lds4, s>>>(d->T, P.J, a2, P.stat, P.tcls, m, P.t2fb, filter);
+    else k_asm2p<1><<<g, kAsmThreads, P.asm2_lds4, s>>>(d->T, P.J, a2, P.stat, P.tcls, m, P.t2fb, filter);
+  }
+  const int n = P.nt - m;
+  if (n <= 0) return;
+  const int *tl = m > 0 ? P.tcls + m : nullptr;
   if (!cache && P.max_L <= 4)
-    k_asm2<0, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
+    k_asm2<0, 7, TPB><<<n, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter, tl);
   else if (!cache)
-    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
+    k_asm2<0, 1, TPB><<<n, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter, tl);
   else if (P.max_lbs <= 3 && P.max_L <= 4)
-    k_asm2<4, 7, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
+    k_asm2<4, 7, TPB><<<n, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter, tl);
   else if (P.max_lbs <= 4 && P.max_L <= 5)
-    k_asm2<5, 9, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
+    k_asm2<5, 9, TPB><<<n, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter, tl);
   else
-    k_asm2<0, 1, TPB><<<P.nt, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter);
+    k_asm2<0, 1, TPB><<<n, TPB, P.asm2_lds, s>>>(d->T, P.J, aa, P.stat, P.t2fb, filter, tl);
 }
 
 // type-1 assembly of the triplets tcls[b .. e) (class-ordered), one per
@@ -2799,13 +2940,15 @@ void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack,
 // so that each triplet's elements finish sooner.
 // spec: the fallback ran speculatively; a triplet failed where an entry
 // failed on both grids
+// The per-class sequence packs the small classes (throughput: the uneven
+// batch's type-2 assembly spends most of its waves on them).
 void launch_asm2(ecp_dev *d, hipStream_t s, int filter, bool fused, bool spec = false) {
   const int mode = d->tune.asm2_cache;
   const bool cache = mode == 1 || (mode == 2 && fused);
   if (fused && filter == 2)
-    launch_asm2_tpb<4 * kAsmThreads>(d, s, filter, cache, spec);
+    launch_asm2_tpb<4 * kAsmThreads>(d, s, filter, cache, spec, false);
   else
-    launch_asm2_tpb<kAsmThreads>(d, s, filter, cache, spec);
+    launch_asm2_tpb<kAsmThreads>(d, s, filter, cache, spec, !fused);
 }
 
 template <int LAB>
@@ -3412,6 +3555,19 @@ extern "C" int ecp_dev_plan(ecp_dev *d, const ecp_job *job, size_t *bytes, char 
       a4.lds_group = std::max(o4, a4.lds_S1 + shift_tab_doubles(LS));
       P.asm1_lds4 = (size_t)kAsmPack * a4.lds_group * sizeof(double);
     }
+    // packed type 2 likewise: blk stays for the shift, whose tables reuse T
+    {
+      AsmArgs &a2 = P.aa2p;
+      const int LS = std::min(LM, kAsmPackLab);
+      a2.shift_lm = LS;
+      int o2 = 0;
+      a2.lds_J = o2; o2 += ECP_DIM_IJK(LS) * ECP_DIM_C(LS);
+      a2.lds_blk = o2; o2 += ECP_DIM_C(LS) * ECP_DIM_C(LS);
+      a2.lds_S2 = o2;
+      a2.lds_T = o2; o2 += (LS + LC) * (LS + LC) * (2 * LS + 1);
+      a2.lds_group = std::max(o2, a2.lds_S2 + shift_tab_doubles(LS));
+      P.asm2_lds4 = (size_t)kAsmPack * a2.lds_group * sizeof(double);
+    }
     P.asm2_lds = (size_t)end2 * sizeof(double);
     if (std::max(P.asm1_lds, P.asm2_lds) > (size_t)d->lds_max) {
       snprintf(err, errlen, "assembly needs %zu bytes of LDS, the device has %d per workgroup",
@@ -3550,7 +3706,7 @@ extern "C" int ecp_dev_bind(ecp_dev *d, void *mem, size_t cap, char *err, size_t
   P.blk = (double *)(A + Ly.d_blk);
   P.zero_base = (unsigned int *)(A + Ly.d_zero);
   P.zero_words = (Ly.d_zero_end - Ly.d_zero) / 4;
-  for (AsmArgs *x : {&P.aa, &P.aa4}) {
+  for (AsmArgs *x : {&P.aa, &P.aa4, &P.aa2p}) {
     AsmArgs &aa = *x;
     aa.units1 = P.u1;
     aa.udir = P.udir;
