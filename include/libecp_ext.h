@@ -163,7 +163,7 @@ int libecp_batch_quadrature_ms(libecp_batch *b, double *ms, int *samples);
    per stage covering every class, with assembly overlapping the large-grid
    fallbacks; 0: per-class launches on forked streams; 2, the default: fused
    for small jobs), "asm2_cache" (type-2 assembly with the angular factors
-   in registers: 0 no, 1 yes, 2, the default: with fused launches),
+   in registers: 0 no, 1, the default: yes, 2 with fused launches only),
    "t2_head" (PS93 levels of the type-2 small-grid kernel before the entries
    still open are rerun compacted; 15, the default: no split; 0: 6 with
    fused launches, no split otherwise), "t2_tail_grid",

@@ -2552,7 +2552,7 @@ struct ecp_dev {
   int *tmpl_dev = nullptr;
   Prepared P;
   JobLayout L;
-  ecp_tuning tune{8, 8, 100000, 8192, 2, 2, 15, 2048, 0, 1, 0};
+  ecp_tuning tune{8, 8, 100000, 8192, 2, 1, 15, 2048, 0, 1, 0};
   bool planned = false, prepared = false, eager_done = false;
   hipGraph_t graph[kSpanGraphs] = {};
   hipGraphExec_t gexec[kSpanGraphs] = {};
@@ -2940,15 +2940,17 @@ void launch_asm1(ecp_dev *d, hipStream_t s, int b, int e, int filter, bool pack,
 // so that each triplet's elements finish sooner.
 // spec: the fallback ran speculatively; a triplet failed where an entry
 // failed on both grids
-// The per-class sequence packs the small classes (throughput: the uneven
-// batch's type-2 assembly spends most of its waves on them).
+// The small classes go four triplets to a wave (they are about half of the
+// triplets: the medium cluster 2.5 % and the uneven batch 0.5 % faster),
+// except in the fused sequence's last assembly, which covers the few
+// triplets with fallback work at four waves each.
 void launch_asm2(ecp_dev *d, hipStream_t s, int filter, bool fused, bool spec = false) {
   const int mode = d->tune.asm2_cache;
   const bool cache = mode == 1 || (mode == 2 && fused);
   if (fused && filter == 2)
     launch_asm2_tpb<4 * kAsmThreads>(d, s, filter, cache, spec, false);
   else
-    launch_asm2_tpb<kAsmThreads>(d, s, filter, cache, spec, !fused);
+    launch_asm2_tpb<kAsmThreads>(d, s, filter, cache, spec, true);
 }
 
 template <int LAB>

@@ -79,8 +79,8 @@ typedef struct {
   int fuse;           /* 1: one launch per stage over all classes, assembly overlapping the
                          large-grid fallbacks; 0: per-class launches on forked streams;
                          2: fused for small jobs, per-class for large ones */
-  int asm2_cache;     /* type-2 assembly with angular factors in registers: 0 no, 1 yes,
-                         2 with fused launches only */
+  int asm2_cache;     /* type-2 assembly with angular factors in registers: 0 no, 1 yes
+                         (the default), 2 with fused launches only */
   int t2_head;        /* PS93 levels of the type-2 head kernel before the compacted tail
                          (>= the level count, the default: no tail; 0: 6 with fused
                          launches, else none) */
