@@ -124,6 +124,7 @@ struct DevJob {
   int FL, UK, angL, ang_lam, ang_c;
   int maxPairs;
   int deriv;  // derivative order: F slabs per run 0..deriv, angular tables to lmax + deriv
+  int ptmajor;  // the type-2 F and U rows point-major (per-class sequence), else row-major
 };
 
 // ---------------------------------------------------------------------------
@@ -360,7 +361,9 @@ __device__ __forceinline__ double dist(const ecp_xyz &a, const ecp_xyz &b) {
 
 // U tables of one (centre, row, chunk of 128 points): row C.L is the local
 // channel U_L (type 1), rows l < L the semi-local channels times r^k,
-// k < UK (type 2), zero past the channel's cut-off end[l].
+// k < UK (type 2), zero past the channel's cut-off end[l].  The type-2 rows
+// are Ut[l][k][n], or with J.ptmajor point-major, Ut[l][n][k]: the lanes of
+// a type-2 group then read the same point of neighbouring k together.
 __device__ void utab_rows(const DevTables &T, const DevJob &J, int c, int row, int chunk) {
   const ecp_centre C = J.cent[c];
   if (row > C.L) return;
@@ -374,15 +377,18 @@ __device__ void utab_rows(const DevTables &T, const DevJob &J, int c, int row, i
   }
   const int l = row;
   double P = (n <= C.end[l]) ? eval_ecp(g, C.ng, l, r) : 0.0;
-  double *dst = J.Ut + C.Ut_off + (size_t)l * J.UK * SO + n;
+  const size_t rs = J.ptmajor ? 1 : SO;  // stride between the powers k
+  double *dst = J.Ut + C.Ut_off + (size_t)l * J.UK * SO + (J.ptmajor ? (size_t)n * J.UK : (size_t)n);
   dst[0] = P;
   for (int k = 1; k < J.UK; k++) {
     P = P * r;
-    dst[(size_t)k * SO] = P;
+    dst[k * rs] = P;
   }
 }
 
-// F[centre][run][shell][lambda][n] (type2.c:263-303), one block per
+// F[centre][run][shell][lambda][n] (type2.c:263-303), or with J.ptmajor
+// F[centre][run][shell][n][lambda] (the lanes of a type-2 group then read
+// the same point of neighbouring lambda together), one block per
 // non-skipped (centre, shell, run); the whole slab is written (zeros outside
 // the window).
 template <int LM>
@@ -410,9 +416,11 @@ __device__ void ftab_slab(const DevTables &T, const DevJob &J, const ecp_centre 
         for (int i = 0; i <= LM; i++) acc[i] += da * K[i] * e;
       }
     }
+    const size_t rs = J.ptmajor ? 1 : SO;  // stride between the orders lambda
+    double *row = slab + (J.ptmajor ? (size_t)n * J.FL : (size_t)n);
 #pragma unroll
-    for (int i = 0; i <= LM; i++) slab[(size_t)i * SO + n] = acc[i];
-    for (int i = LM + 1; i < J.FL; i++) slab[(size_t)i * SO + n] = 0.0;
+    for (int i = 0; i <= LM; i++) row[i * rs] = acc[i];
+    for (int i = LM + 1; i < J.FL; i++) row[i * rs] = 0.0;
   }
 }
 
@@ -1161,13 +1169,15 @@ __device__ __forceinline__ int t2_entry(const DevTables &T, const DevJob &J, con
   const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
   const ecp_triplet tr = J.trip[G.trip];
   const ecp_centre C = J.cent[tr.centre];
+  // point and row strides of the F and U rows (layouts as the tables kernel writes them)
+  const int pF = J.ptmajor ? J.FL : 1, pU = J.ptmajor ? J.UK : 1, rs = J.ptmajor ? 1 : SO;
   const double *__restrict__ Fa =
-      J.F + C.F_off + (((size_t)tr.powa * C.nsh + (tr.sa - C.shell0)) * J.FL + l1) * SO;
+      J.F + C.F_off + ((size_t)tr.powa * C.nsh + (tr.sa - C.shell0)) * J.FL * SO + (size_t)l1 * rs;
   const double *__restrict__ Fb =
-      J.F + C.F_off + (((size_t)tr.powb * C.nsh + (tr.sb - C.shell0)) * J.FL + l2) * SO;
-  const double *__restrict__ Uu = J.Ut + C.Ut_off + ((size_t)G.l * J.UK + l3) * SO;
+      J.F + C.F_off + ((size_t)tr.powb * C.nsh + (tr.sb - C.shell0)) * J.FL * SO + (size_t)l2 * rs;
+  const double *__restrict__ Uu = J.Ut + C.Ut_off + (size_t)G.l * J.UK * SO + (size_t)l3 * rs;
   const int gs = tr.gs, ge = tr.ge;
-  auto f = [&](int n) { return T.ws[n] * ((Fa[n] * Fb[n]) * Uu[n]); };
+  auto f = [&](int n) { return T.ws[n] * ((Fa[n * pF] * Fb[n * pF]) * Uu[n * pU]); };
   double p = f(T.s93.centre);
   double q = f(T.s93.first_l) + f(T.s93.first_r);
   double I = p + q;
@@ -1295,13 +1305,15 @@ __global__ __launch_bounds__(64) void k_type2_tail(DevTables T, DevJob J, T2Args
     const int l1 = code & 0xff, l2 = (code >> 8) & 0xff, l3 = (code >> 16) & 0xff;
     const ecp_triplet tr = J.trip[G.trip];
     const ecp_centre C = J.cent[tr.centre];
+    // point and row strides of the F and U rows (layouts as the tables kernel writes them)
+    const int pF = J.ptmajor ? J.FL : 1, pU = J.ptmajor ? J.UK : 1, rs = J.ptmajor ? 1 : SO;
     const double *__restrict__ Fa =
-        J.F + C.F_off + (((size_t)tr.powa * C.nsh + (tr.sa - C.shell0)) * J.FL + l1) * SO;
+        J.F + C.F_off + ((size_t)tr.powa * C.nsh + (tr.sa - C.shell0)) * J.FL * SO + (size_t)l1 * rs;
     const double *__restrict__ Fb =
-        J.F + C.F_off + (((size_t)tr.powb * C.nsh + (tr.sb - C.shell0)) * J.FL + l2) * SO;
-    const double *__restrict__ Uu = J.Ut + C.Ut_off + ((size_t)G.l * J.UK + l3) * SO;
+        J.F + C.F_off + ((size_t)tr.powb * C.nsh + (tr.sb - C.shell0)) * J.FL * SO + (size_t)l2 * rs;
+    const double *__restrict__ Uu = J.Ut + C.Ut_off + (size_t)G.l * J.UK * SO + (size_t)l3 * rs;
     const int gs = tr.gs, ge = tr.ge;
-    auto f = [&](int n) { return T.ws[n] * ((Fa[n] * Fb[n]) * Uu[n]); };
+    auto f = [&](int n) { return T.ws[n] * ((Fa[n * pF] * Fb[n * pF]) * Uu[n * pU]); };
     double p = 0.0, q = 0.0, I = 0.0, res = 0.0;
     bool done = !live;
     if (live && r == 0) {
@@ -3100,7 +3112,12 @@ void enqueue_t1_class(ecp_dev *d, hipStream_t ws, int c) {
 
 void enqueue(ecp_dev *d, hipStream_t s, hipEvent_t *evs) {
   const int units = d->P.cls_begin[ECP_MAX_LAB + 1];
-  if (d->tune.fuse == 1 || (d->tune.fuse == 2 && units <= kFuseAutoMaxUnits)) {
+  const bool fused = d->tune.fuse == 1 || (d->tune.fuse == 2 && units <= kFuseAutoMaxUnits);
+  // point-major type-2 rows in the throughput-bound per-class sequence (the
+  // uneven batch 1.8 % faster); row-major in the fused one, where the
+  // point-major tables cost the medium cluster 2.7 %
+  d->P.J.ptmajor = fused ? 0 : 1;
+  if (fused) {
     enqueue_fused(d, s, evs);
     return;
   }
