@@ -130,12 +130,17 @@ struct DevJob {
 // ---------------------------------------------------------------------------
 // numerical primitives
 
+// Radial powers other than 0, 1, 2 (rare in ECPs) through a call: inlined,
+// pow's registers set the fallbacks' budget (the type-1 fallback 152
+// registers and three waves per SIMD, against 128 and four)
+__device__ __attribute__((noinline)) double ecp_pow_general(double r, double n) { return pow(r, n); }
+
 __device__ __forceinline__ double ecp_pow(double r, double n) {
   // exact special cases of a correctly rounded pow
   if (n == 2.0) return r * r;
   if (n == 1.0) return r;
   if (n == 0.0) return 1.0;
-  return pow(r, n);
+  return ecp_pow_general(r, n);
 }
 
 __device__ double eval_ecp(const ecp_gauss *__restrict__ g, int ng, int l, double r) {
