@@ -86,6 +86,29 @@ def test_random_molecule_against_oracle(seed):
         assert ok, (m.name, n, k, 1 + i % 2, excess)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_general_radial_powers_against_oracle(seed):
+    """ECP radial powers outside 0, 1, 2 (0.5, 1.5, 3): the device takes
+    them through its general pow, the oracle through C's pow."""
+    m, n = random_molecule(seed)
+    rng = np.random.default_rng(7000 + seed)
+    m.nECP = rng.choice([0.5, 1.5, 2.0, 3.0], size=m.nECP.shape).astype(np.float64)
+    assert np.any(m.nECP != 2.0)
+    h = native.Handle(m, n=n)
+    try:
+        rc, keys, blocks = h.blocks()
+    finally:
+        h.close()
+    rc2, col = E.oracle_blocks(m, n=n, options=1)
+    assert rc == rc2
+    assert keys == col.keys
+    for i, (k, a, b) in enumerate(zip(keys, blocks, col.blocks)):
+        if not np.isfinite(b).all():
+            continue
+        ok, excess = within(a, b, ATOL, RTOL)
+        assert ok, (m.name, n, k, 1 + i % 2, excess)
+
+
 def test_random_batch_equals_molecules_alone():
     mols = [random_molecule(s)[0] for s in range(12)]
     b = native.Batch(mols)
