@@ -2034,7 +2034,7 @@ __global__ __launch_bounds__(kAsmThreads, 3) void k_asm1(DevTables T, DevJob J, 
         return;
       }
     } else {
-      // the verdict of the lane group's W lanes
+      // the decision of the lane group's W lanes
       const unsigned long long m = __ballot(bad);
       if ((m >> (grp * W)) & ((1ull << W) - 1ull)) {
         if (lid == 0) status[2 * t] = 1;
@@ -2296,7 +2296,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm2p(DevTables T, DevJob J, As
   if (A.fbf2 && filter == 2) {
     // speculative fallback: the triplet failed where an entry failed on the
     // small grid and on the large grid (type2.c:417-528, code 1); the
-    // verdict of the group's lanes
+    // decision of the group's lanes
     bool bad = false;
     if (active)
       for (int l = 0; l < Cn.L; l++) {
